@@ -1,0 +1,1688 @@
+// engine.hip — oxDNA force engine for gfx950: kernels, context, C ABI.
+//
+// Data layout in HBM (N particles, one context):
+//   pos    double4[N]  x, y, z, type id in the low 32 bits of .w (host-engine layout)
+//   quat   double4[N]  w, x, y, z
+//   vel    double4[N], angmom double4[N] (quaternion conjugate momentum, NO_SQUISH)
+//   frames float4[3][N] a1 (+charge), a2 (+type), a3 — rebuilt every step
+//   force, torque float4[N] (force.w = half energy of i), virial float[6][N]
+//   nlist  uint32[N][max_neigh] rows sorted by index, n_neigh uint32[N]
+//   cells  uint32[ncell] counts + uint32[ncell][cell_cap] members
+// One step = 4 kernels on one stream (captured as a HIP graph):
+//   step1 (half kick, drift, NO_SQUISH rotation, frames, cell binning, skin test)
+//   -> nlist build (exits at once unless the skin test fired)
+//   -> fused forces (every selected term + bonded in one neighbour traversal)
+//   -> step2 (Langevin half kick, translational + rotational).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "oxdna_hip.h"
+
+using namespace oxd;
+
+// =====================================================================================
+// kernels
+// =====================================================================================
+namespace oxd {
+
+__global__ void k_frames(uint32_t N, const double4* __restrict__ quat, const double* __restrict__ charge,
+                         const double4* __restrict__ pos, float4* fr0, float4* fr1, float4* fr2) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double4 q = quat[i];
+    float4 a, b, c;
+    quat_axes(q.x, q.y, q.z, q.w, a, b, c);
+    a.w = charge ? (float)charge[i] : 1.0f;
+    b.w = __int_as_float(type_of(pos[i].w));
+    c.w = 0.f;
+    fr0[i] = a; fr1[i] = b; fr2[i] = c;
+}
+
+__device__ __forceinline__ uint32_t cell_index(double x, double y, double z, const double* Linv, const int* nc) {
+    double u[3] = {x * Linv[0] + 0.5, y * Linv[1] + 0.5, z * Linv[2] + 0.5};
+    int c[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double v = u[a] - floor(u[a]);
+        int k = (int)(v * nc[a]);
+        c[a] = k >= nc[a] ? nc[a] - 1 : (k < 0 ? 0 : k);
+    }
+    return ((uint32_t)c[2] * nc[1] + c[1]) * nc[0] + c[0];
+}
+
+struct CellArgs {
+    uint32_t N;
+    const double4* pos;
+    double Linv[3];
+    int nc[3];
+    uint32_t cap;
+    uint32_t* count;
+    uint32_t* members;
+    DevFlags* flags;
+};
+
+__device__ __forceinline__ void bin_particle(const CellArgs& a, uint32_t i, double x, double y, double z) {
+    uint32_t c = cell_index(x, y, z, a.Linv, a.nc);
+    uint32_t slot = atomicAdd(&a.count[c], 1u);
+    if (slot < a.cap) a.members[(size_t)c * a.cap + slot] = i;
+    else atomicOr(&a.flags->overflow, 1u);
+    atomicMax(&a.flags->max_cell, slot + 1);
+}
+
+__global__ void k_cell_assign(CellArgs a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N) return;
+    double4 p = a.pos[i];
+    bin_particle(a, i, p.x, p.y, p.z);
+}
+
+__global__ void k_zero_u32(uint32_t* p, size_t n) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) p[k] = 0u;
+}
+
+struct NlistArgs {
+    uint32_t N;
+    const double4* pos;
+    double L[3], Linv[3];
+    int nc[3];
+    uint32_t cap;
+    const uint32_t* count;
+    const uint32_t* members;
+    const int* bpart;
+    double rlist2;
+    uint32_t max_neigh;
+    uint32_t* n_neigh;
+    uint32_t* nlist;
+    double4* pos_last;
+    DevFlags* flags;
+    int conditional;  // 1: only when flags->rebuild is set
+};
+
+// One thread per particle: scan the (up to) 27 cells around i, keep j != i with
+// |dr|^2 < r_list^2 (FP64, no contraction) that is not bonded to i, then sort the row.
+__global__ void k_nlist_build(NlistArgs a) {
+    if (a.conditional && a.flags->rebuild == 0u) return;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N) return;
+    if (i == 0) atomicAdd(&a.flags->builds, 1ull);
+    double4 pi = a.pos[i];
+    a.pos_last[i] = pi;
+    int b0 = a.bpart[2 * i], b1 = a.bpart[2 * i + 1];
+    uint32_t c = cell_index(pi.x, pi.y, pi.z, a.Linv, a.nc);
+    int cx = c % a.nc[0], cy = (c / a.nc[0]) % a.nc[1], cz = c / (a.nc[0] * a.nc[1]);
+    int lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = a.nc[d] >= 3 ? -1 : 0;
+        hi[d] = a.nc[d] >= 3 ? 1 : (a.nc[d] == 2 ? 1 : 0);
+    }
+    uint32_t* row = a.nlist + (size_t)i * a.max_neigh;
+    uint32_t n = 0;
+    for (int dz = lo[2]; dz <= hi[2]; ++dz)
+        for (int dy = lo[1]; dy <= hi[1]; ++dy)
+            for (int dx = lo[0]; dx <= hi[0]; ++dx) {
+                int x = (cx + dx + a.nc[0]) % a.nc[0];
+                int y = (cy + dy + a.nc[1]) % a.nc[1];
+                int z = (cz + dz + a.nc[2]) % a.nc[2];
+                uint32_t cc = ((uint32_t)z * a.nc[1] + y) * a.nc[0] + x;
+                uint32_t m = min(a.count[cc], a.cap);
+                const uint32_t* mem = a.members + (size_t)cc * a.cap;
+                for (uint32_t k = 0; k < m; ++k) {
+                    uint32_t j = mem[k];
+                    if (j == i || (int)j == b0 || (int)j == b1) continue;
+                    double4 pj = a.pos[j];
+                    double ddx = min_image(pi.x - pj.x, a.L[0], a.Linv[0]);
+                    double ddy = min_image(pi.y - pj.y, a.L[1], a.Linv[1]);
+                    double ddz = min_image(pi.z - pj.z, a.L[2], a.Linv[2]);
+                    double r2;
+                    {
+#pragma clang fp contract(off)
+                        r2 = ddx * ddx;
+                        r2 = r2 + ddy * ddy;
+                        r2 = r2 + ddz * ddz;
+                    }
+                    if (r2 < a.rlist2) {
+                        if (n < a.max_neigh) row[n] = j;
+                        ++n;
+                    }
+                }
+            }
+    a.n_neigh[i] = min(n, a.max_neigh);
+    if (n > a.max_neigh) atomicOr(&a.flags->overflow, 2u);
+    atomicMax(&a.flags->max_neigh, n);
+    // insertion sort of the row: deterministic summation order
+    uint32_t m = min(n, a.max_neigh);
+    for (uint32_t p = 1; p < m; ++p) {
+        uint32_t v = row[p];
+        int q = (int)p - 1;
+        while (q >= 0 && row[q] > v) { row[q + 1] = row[q]; --q; }
+        row[q + 1] = v;
+    }
+}
+
+// ------------------------------------------------------------------ fused forces
+template<int TPP, bool WANT_E, bool WANT_V, bool COUNT = false>
+__global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.cell_count) {
+        size_t stride = (size_t)gridDim.x * blockDim.x;
+        for (size_t c = tid; c < a.ncell_total; c += stride) a.cell_count[c] = 0u;
+    }
+    const uint32_t i = tid / TPP;
+    const uint32_t lane = tid % TPP;
+    const bool active = i < a.N;
+    const DParams& P = *a.P;
+    const uint32_t mask = a.mask;
+    V F = zero3(), T = zero3();
+    float Eh = 0.f;
+    float et[8];
+    float vir[6];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) et[s] = 0.f;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) vir[s] = 0.f;
+    uint32_t cnt[14];
+#pragma unroll
+    for (int s = 0; s < 14; ++s) cnt[s] = 0u;
+    if (active) {
+        const double4 pi = a.pos[i];
+        const float4 f0 = a.fr0[i], f1 = a.fr1[i], f2 = a.fr2[i];
+        Frame fi{mk(f0.x, f0.y, f0.z), mk(f1.x, f1.y, f1.z), mk(f2.x, f2.y, f2.z)};
+        const float qi = f0.w;
+        const int ti = __float_as_int(f1.w);
+        const int nt = P.ntypes;
+        const uint32_t nn = a.n_neigh[i];
+        const size_t h = a.head[i];
+        for (uint32_t k = lane; k < nn; k += TPP) {
+            const uint32_t j = a.nlist[h + k];
+            const double4 pj = a.pos[j];
+            const float4 g0 = a.fr0[j], g1 = a.fr1[j], g2 = a.fr2[j];
+            Frame fj{mk(g0.x, g0.y, g0.z), mk(g1.x, g1.y, g1.z), mk(g2.x, g2.y, g2.z)};
+            const int tj = __float_as_int(g1.w);
+            const int tp = ti * nt + tj;
+            V dr = mk((float)min_image(pi.x - pj.x, a.L[0], a.Linv[0]),
+                      (float)min_image(pi.y - pj.y, a.L[1], a.Linv[1]),
+                      (float)min_image(pi.z - pj.z, a.L[2], a.Linv[2]));
+            const float rsq = dot(dr, dr);
+            const V Rc = -dr;
+            V Fp = zero3();
+            float ep = 0.f;
+            if ((mask & (1u << T_EXCVOL)) && rsq < P.rcutsq[T_EXCVOL][tp]) {
+                float e = ev_pair(Rc, fi, fj, P.ev, Fp, T);
+                et[S_NEXC] += e; ep += e;
+                if (COUNT) { cnt[2 * T_EXCVOL]++; cnt[2 * T_EXCVOL + 1] += e != 0.f; }
+            }
+            if ((mask & (1u << T_HBOND)) && rsq < P.rcutsq[T_HBOND][tp]) {
+                BB g;
+                if (bb_geom(Rc, fi, fj, P.hb.base, g)) {
+                    float eps = P.hb.hb_eps[tp], rv, rd;
+                    f1_vd(g.rmod, P.hb.f1, eps, eps * P.hb.shift_factor, rv, rd);
+                    float e = bb_term(g, fi, fj, rv, rd, P.hb.hb_t1, P.hb.hb_t23, P.hb.hb_t4, P.hb.hb_t78,
+                                      false, true, Fp, T);
+                    et[S_HB] += e; ep += e;
+                    if (COUNT) cnt[2 * T_HBOND + 1] += e != 0.f;
+                }
+                if (COUNT) cnt[2 * T_HBOND]++;
+            }
+            if ((mask & (1u << T_CROSS)) && rsq < P.rcutsq[T_CROSS][tp]) {
+                BB g;
+                if (bb_geom(Rc, fi, fj, P.cs.base, g)) {
+                    float rv, rd;
+                    f2_vd(g.rmod, P.cs.f2, rv, rd);
+                    float e = bb_term(g, fi, fj, rv, rd, P.cs.cs_t1, P.cs.cs_t23, P.cs.cs_t4, P.cs.cs_t78,
+                                      true, P.cs.cs_t4_enabled != 0, Fp, T);
+                    et[S_CR] += e; ep += e;
+                    if (COUNT) cnt[2 * T_CROSS + 1] += e != 0.f;
+                }
+                if (COUNT) cnt[2 * T_CROSS]++;
+            }
+            if ((mask & (1u << T_HBONDCROSS)) && rsq < P.rcutsq[T_HBONDCROSS][tp]) {
+                BB g;
+                if (bb_geom(Rc, fi, fj, P.hbx.base, g)) {
+                    float eps = P.hbx.hb_eps[tp], rv, rd;
+                    f1_vd(g.rmod, P.hbx.f1, eps, eps * P.hbx.shift_factor, rv, rd);
+                    float e1 = bb_term(g, fi, fj, rv, rd, P.hbx.hb_t1, P.hbx.hb_t23, P.hbx.hb_t4, P.hbx.hb_t78,
+                                       false, true, Fp, T);
+                    f2_vd(g.rmod, P.hbx.f2, rv, rd);
+                    float e2 = bb_term(g, fi, fj, rv, rd, P.hbx.cs_t1, P.hbx.cs_t23, P.hbx.cs_t4, P.hbx.cs_t78,
+                                       true, P.hbx.cs_t4_enabled != 0, Fp, T);
+                    et[S_HB] += e1; et[S_CR] += e2; ep += e1 + e2;
+                    if (COUNT) cnt[2 * T_HBONDCROSS + 1] += (e1 != 0.f || e2 != 0.f);
+                }
+                if (COUNT) cnt[2 * T_HBONDCROSS]++;
+            }
+            if ((mask & (1u << T_COAX)) && rsq < P.rcutsq[T_COAX][tp]) {
+                float e = coax_pair(Rc, fi, fj, P.cx, Fp, T);
+                et[S_CX] += e; ep += e;
+                if (COUNT) { cnt[2 * T_COAX]++; cnt[2 * T_COAX + 1] += e != 0.f; }
+            }
+            if ((mask & (1u << T_DEBYE)) && rsq < P.rcutsq[T_DEBYE][tp]) {
+                float e = dh_pair(Rc, fi, fj, qi * g0.w, P.dh, Fp, T);
+                et[S_DH] += e; ep += e;
+                if (COUNT) { cnt[2 * T_DEBYE]++; cnt[2 * T_DEBYE + 1] += e != 0.f; }
+            }
+            F += Fp;
+            Eh += 0.5f * ep;
+            if (WANT_V) {
+                vir[0] += 0.5f * dr.x * Fp.x; vir[1] += 0.5f * dr.x * Fp.y; vir[2] += 0.5f * dr.x * Fp.z;
+                vir[3] += 0.5f * dr.y * Fp.y; vir[4] += 0.5f * dr.y * Fp.z; vir[5] += 0.5f * dr.z * Fp.z;
+            }
+        }
+        if (mask & (1u << T_BONDED)) {
+            for (uint32_t b = lane; b < 2; b += TPP) {
+                const int other = a.bpart[2 * i + b];
+                if (other < 0) continue;
+                const int info = a.binfo[2 * i + b];
+                const int role = info & 1;
+                const int bt = info >> 1;
+                const uint32_t A = role == 0 ? i : (uint32_t)other;
+                const uint32_t B = role == 0 ? (uint32_t)other : i;
+                const double4 pA = a.pos[A], pB = a.pos[B];
+                const float4 x0 = a.fr0[A], x1 = a.fr1[A], x2 = a.fr2[A];
+                const float4 y0 = a.fr0[B], y1 = a.fr1[B], y2 = a.fr2[B];
+                Frame fA{mk(x0.x, x0.y, x0.z), mk(x1.x, x1.y, x1.z), mk(x2.x, x2.y, x2.z)};
+                Frame fB{mk(y0.x, y0.y, y0.z), mk(y1.x, y1.y, y1.z), mk(y2.x, y2.y, y2.z)};
+                BondRes r;
+                bond_eval(P.bond[bt], min_image(pB.x - pA.x, a.L[0], a.Linv[0]),
+                          min_image(pB.y - pA.y, a.L[1], a.Linv[1]), min_image(pB.z - pA.z, a.L[2], a.Linv[2]),
+                          fA, fB, __float_as_int(x1.w), __float_as_int(y1.w), r);
+                if (COUNT) { cnt[2 * T_BONDED]++; cnt[2 * T_BONDED + 1] += r.e_stack != 0.f; }
+                if (role == 0) { F -= r.FB; T += r.TA; }
+                else { F += r.FB; T += r.TB; }
+                // per-term sums count every directed evaluation in full and are halved at
+                // the end (each pair and each bond is evaluated from both of its ends)
+                float ef = (float)r.e_fene;
+                et[S_FENE] += ef;
+                et[S_BEXC] += r.e_bexc;
+                et[S_STACK] += r.e_stack;
+                Eh += 0.5f * (ef + r.e_bexc + r.e_stack);
+                if (WANT_V) {
+#pragma unroll
+                    for (int s = 0; s < 6; ++s) vir[s] += r.vir[s];
+                }
+            }
+        }
+    }
+    if (COUNT) {
+#pragma unroll
+        for (int s = 0; s < 14; ++s)
+            if (cnt[s]) atomicAdd(&a.counts[s], (unsigned long long)cnt[s]);
+    }
+    // reduce the TPP lanes of a particle (they are adjacent lanes of one wavefront)
+#pragma unroll
+    for (int m = TPP / 2; m > 0; m >>= 1) {
+        F.x += __shfl_xor(F.x, m); F.y += __shfl_xor(F.y, m); F.z += __shfl_xor(F.z, m);
+        T.x += __shfl_xor(T.x, m); T.y += __shfl_xor(T.y, m); T.z += __shfl_xor(T.z, m);
+        Eh += __shfl_xor(Eh, m);
+        if (WANT_V) {
+#pragma unroll
+            for (int s = 0; s < 6; ++s) vir[s] += __shfl_xor(vir[s], m);
+        }
+    }
+    if (a.split) {
+        float s0 = et[S_FENE], s1 = et[S_BEXC], s2 = et[S_STACK];
+#pragma unroll
+        for (int m = TPP / 2; m > 0; m >>= 1) {
+            s0 += __shfl_xor(s0, m); s1 += __shfl_xor(s1, m); s2 += __shfl_xor(s2, m);
+        }
+        if (active && lane == 0) a.split[i] = make_double4(0.5 * s0, 0.5 * s1, 0.5 * s2, 0.0);
+    }
+    if (active && lane == 0) {
+        float4 fo = make_float4(F.x, F.y, F.z, Eh);
+        float4 to = make_float4(T.x, T.y, T.z, 0.f);
+        if (a.accumulate) {
+            float4 f = a.force[i], t = a.torque[i];
+            fo.x += f.x; fo.y += f.y; fo.z += f.z; fo.w += f.w;
+            to.x += t.x; to.y += t.y; to.z += t.z;
+        }
+        a.force[i] = fo;
+        a.torque[i] = to;
+        if (WANT_V) {
+#pragma unroll
+            for (int s = 0; s < 6; ++s) {
+                float v = vir[s];
+                if (a.accumulate) v += a.virial[s * a.vpitch + i];
+                a.virial[s * a.vpitch + i] = v;
+            }
+        }
+    }
+    if (WANT_E) {
+        // per-term totals: FP64 from here on (wave shuffle, then across waves in LDS)
+        __shared__ double red[256 / 64][8];
+        const int wid = threadIdx.x / 64, wl = threadIdx.x % 64;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            double v = (double)et[s];
+#pragma unroll
+            for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+            if (wl == 0) red[wid][s] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 8) {
+            double v = 0.0;
+            for (int w = 0; w < (int)(blockDim.x / 64); ++w) v += red[w][threadIdx.x];
+            a.epart[(size_t)blockIdx.x * 8 + threadIdx.x] = 0.5 * v;  // full list: each pair seen twice
+        }
+    }
+}
+
+// Sums per-block partial energies into out[8] (one block; fixed order).
+__global__ void k_reduce_energy(uint32_t nblocks, const double* __restrict__ part, double* out, int accumulate) {
+    __shared__ double s[256];
+    for (int slot = 0; slot < 8; ++slot) {
+        double v = 0.0;
+        for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x) v += part[(size_t)b * 8 + slot];
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[slot] = (accumulate ? out[slot] : 0.0) + s[0];
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ integrator
+// Counter-based random numbers: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+        uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+}
+__device__ __forceinline__ double u01(uint32_t x) { return ((double)x + 0.5) * (1.0 / 4294967296.0); }
+
+// q1 (x) q2 for quaternions stored (w, x, y, z) in double4 (x=w, y=x, z=y, w=z).
+__device__ __forceinline__ double4 qmul(double4 a, double4 b) {
+    return make_double4(a.x * b.x - a.y * b.y - a.z * b.z - a.w * b.w,
+                        a.x * b.y + a.y * b.x + a.z * b.w - a.w * b.z,
+                        a.x * b.z - a.y * b.w + a.z * b.x + a.w * b.y,
+                        a.x * b.w + a.y * b.z - a.z * b.y + a.w * b.x);
+}
+__device__ __forceinline__ double4 qconj(double4 a) { return make_double4(a.x, -a.y, -a.z, -a.w); }
+// rotate the lab vector v into the body frame: conj(q) v q
+__device__ __forceinline__ void to_body(double4 q, double vx, double vy, double vz, double& bx, double& by, double& bz) {
+    double4 t = qmul(qmul(qconj(q), make_double4(0.0, vx, vy, vz)), q);
+    bx = t.y; by = t.z; bz = t.w;
+}
+__device__ __forceinline__ void to_lab(double4 q, double vx, double vy, double vz, double& bx, double& by, double& bz) {
+    double4 t = qmul(qmul(q, make_double4(0.0, vx, vy, vz)), qconj(q));
+    bx = t.y; by = t.z; bz = t.w;
+}
+// Free-rotor sub-step about body axis k for time tau (Miller et al., J. Chem. Phys.
+// 116, 8649 (2002), NO_SQUISH): P_k permutes quaternion components.
+__device__ __forceinline__ double4 permute(int k, double4 q) {
+    if (k == 1) return make_double4(-q.y, q.x, q.w, -q.z);
+    if (k == 2) return make_double4(-q.z, -q.w, q.x, q.y);
+    return make_double4(-q.w, q.z, -q.y, q.x);
+}
+__device__ __forceinline__ void rotor(int k, double tau, double Ik, double4& q, double4& p) {
+    double4 Pq = permute(k, q), Pp = permute(k, p);
+    double zeta = (p.x * Pq.x + p.y * Pq.y + p.z * Pq.z + p.w * Pq.w) / (4.0 * Ik);
+    double c = cos(zeta * tau), s = sin(zeta * tau);
+    q = make_double4(c * q.x + s * Pq.x, c * q.y + s * Pq.y, c * q.z + s * Pq.z, c * q.w + s * Pq.w);
+    p = make_double4(c * p.x + s * Pp.x, c * p.y + s * Pp.y, c * p.z + s * Pp.z, c * p.w + s * Pp.w);
+}
+
+struct StepArgs {
+    uint32_t N;
+    double4* bd;       // Langevin force (xyz) of the last second half-kick, lab frame
+    double4* bd_tq;    // Langevin torque (xyz) of the last second half-kick, lab frame
+    double4* pos;
+    double4* vel;
+    double4* quat;
+    double4* angmom;
+    const float4* force;
+    const float4* torque;
+    const double* charge;
+    float4* fr0;
+    float4* fr1;
+    float4* fr2;
+    const double4* pos_last;
+    double dt, kT, gamma, gamma_r, mass, inertia;
+    double L[3], Linv[3];
+    double skin_half2;
+    unsigned long long seed;
+    DevFlags* flags;
+    CellArgs cells;
+};
+
+// First half of the step: v += dt/2 (F + F_L)/m, x += dt v (wrapped),
+// p += dt q (x) (tau + tau_L)_body, where F_L / tau_L are the Langevin force and torque
+// drawn in the previous second half (velocity Verlet on the total force, as the
+// host engine's Langevin method does),
+// NO_SQUISH rotation (3: dt/2, 2: dt/2, 1: dt, 2: dt/2, 3: dt/2), renormalised q; then
+// the new frames, cell binning, and the skin test that requests a rebuild.
+__global__ void k_step1(StepArgs a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) a.flags->step += 1ull;
+    if (i >= a.N) return;
+    const double dt = a.dt;
+    double4 x = a.pos[i], v = a.vel[i];
+    float4 f = a.force[i];
+    const double4 bd = a.bd[i], bdt = a.bd_tq[i];
+    const double hm = 0.5 * dt / a.mass;
+    v.x += hm * (f.x + bd.x); v.y += hm * (f.y + bd.y); v.z += hm * (f.z + bd.z);
+    x.x += dt * v.x; x.y += dt * v.y; x.z += dt * v.z;
+    x.x -= a.L[0] * floor(x.x * a.Linv[0] + 0.5);
+    x.y -= a.L[1] * floor(x.y * a.Linv[1] + 0.5);
+    x.z -= a.L[2] * floor(x.z * a.Linv[2] + 0.5);
+    double4 q = a.quat[i], p = a.angmom[i];
+    float4 t = a.torque[i];
+    double tbx, tby, tbz;
+    to_body(q, t.x + bdt.x, t.y + bdt.y, t.z + bdt.z, tbx, tby, tbz);
+    double4 kick = qmul(q, make_double4(0.0, tbx, tby, tbz));
+    p.x += dt * kick.x; p.y += dt * kick.y; p.z += dt * kick.z; p.w += dt * kick.w;
+    const double I = a.inertia;
+    rotor(3, 0.5 * dt, I, q, p);
+    rotor(2, 0.5 * dt, I, q, p);
+    rotor(1, dt, I, q, p);
+    rotor(2, 0.5 * dt, I, q, p);
+    rotor(3, 0.5 * dt, I, q, p);
+    double nq = 1.0 / sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    q.x *= nq; q.y *= nq; q.z *= nq; q.w *= nq;
+    a.pos[i] = x; a.vel[i] = v; a.quat[i] = q; a.angmom[i] = p;
+    float4 f0, f1, f2;
+    quat_axes(q.x, q.y, q.z, q.w, f0, f1, f2);
+    f0.w = a.charge ? (float)a.charge[i] : 1.f;
+    f1.w = __int_as_float(type_of(x.w));
+    f2.w = 0.f;
+    a.fr0[i] = f0; a.fr1[i] = f1; a.fr2[i] = f2;
+    bin_particle(a.cells, i, x.x, x.y, x.z);
+    double4 l = a.pos_last[i];
+    double dx = min_image(x.x - l.x, a.L[0], a.Linv[0]);
+    double dy = min_image(x.y - l.y, a.L[1], a.Linv[1]);
+    double dz = min_image(x.z - l.z, a.L[2], a.Linv[2]);
+    if (dx * dx + dy * dy + dz * dz > a.skin_half2) atomicOr(&a.flags->rebuild, 1u);
+}
+
+// Second half: Langevin forces (drag -gamma v plus uniform noise of variance
+// 2 gamma kT / dt per component; rotational drag -gamma_r L/I plus Gaussian noise of
+// variance 2 gamma_r kT / dt in the body frame), then v += dt/2 F/m, p += dt q (x) tau.
+__global__ void k_step2(StepArgs a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) a.flags->rebuild = 0u;
+    if (i >= a.N) return;
+    const double dt = a.dt;
+    const unsigned long long step = a.flags->step;
+    uint32_t c[4] = {i, (uint32_t)step, (uint32_t)(step >> 32), 0u};
+    philox(c, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
+    uint32_t d[4] = {i, (uint32_t)step, (uint32_t)(step >> 32), 1u};
+    philox(d, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
+    double4 v = a.vel[i];
+    float4 f = a.force[i];
+    const double ct = sqrt(6.0 * a.gamma * a.kT / dt);
+    double bx = -a.gamma * v.x + ct * (2.0 * u01(c[0]) - 1.0);
+    double by = -a.gamma * v.y + ct * (2.0 * u01(c[1]) - 1.0);
+    double bz = -a.gamma * v.z + ct * (2.0 * u01(c[2]) - 1.0);
+    const double hm = 0.5 * dt / a.mass;
+    v.x += hm * (f.x + bx); v.y += hm * (f.y + by); v.z += hm * (f.z + bz);
+    a.vel[i] = v;
+    a.bd[i] = make_double4(bx, by, bz, 0.0);
+    double4 q = a.quat[i], p = a.angmom[i];
+    float4 t = a.torque[i];
+    double tbx, tby, tbz;
+    to_body(q, t.x, t.y, t.z, tbx, tby, tbz);
+    double4 Lb = qmul(qconj(q), p);  // = 2 (0, L_body)
+    const double I = a.inertia;
+    const double cr = sqrt(2.0 * a.gamma_r * a.kT / dt);
+    double r1 = sqrt(-2.0 * log(u01(c[3]))), r2 = sqrt(-2.0 * log(u01(d[1])));
+    double g0 = r1 * cos(6.283185307179586 * u01(d[0]));
+    double g1 = r1 * sin(6.283185307179586 * u01(d[0]));
+    double g2 = r2 * cos(6.283185307179586 * u01(d[2]));
+    double rbx = cr * g0 - a.gamma_r * (0.5 * Lb.y) / I;
+    double rby = cr * g1 - a.gamma_r * (0.5 * Lb.z) / I;
+    double rbz = cr * g2 - a.gamma_r * (0.5 * Lb.w) / I;
+    tbx += rbx; tby += rby; tbz += rbz;
+    double lx, ly, lz;
+    to_lab(q, rbx, rby, rbz, lx, ly, lz);
+    a.bd_tq[i] = make_double4(lx, ly, lz, 0.0);
+    double4 kick = qmul(q, make_double4(0.0, tbx, tby, tbz));
+    p.x += dt * kick.x; p.y += dt * kick.y; p.z += dt * kick.z; p.w += dt * kick.w;
+    a.angmom[i] = p;
+}
+
+}  // namespace oxd
+
+// =====================================================================================
+// context
+// =====================================================================================
+namespace {
+
+thread_local std::string g_error;
+
+struct Fail : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIP_OK(call)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) throw Fail(std::string(#call) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+template<class T> struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        if (count <= n && p) return;
+        release();
+        if (count == 0) return;
+        HIP_OK(hipMalloc((void**)&p, count * sizeof(T)));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+constexpr int kBlock = 256;
+constexpr int kTPP = 8;
+
+}  // namespace
+
+struct oxdna_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+
+    uint32_t N = 0;
+    double L[3] = {0, 0, 0}, Linv[3] = {0, 0, 0};
+    DBuf<double4> pos, vel, quat, angmom, pos_last, bd, bd_tq;
+    DBuf<double> charge;
+    DBuf<float4> fr0, fr1, fr2, force, torque;
+    DBuf<float> virial;
+    DBuf<int> bpart, binfo;
+    DBuf<uint32_t> n_neigh, nlist, cell_count, cell_members;
+    DBuf<size_t> head;
+    DBuf<double> epart, energy;
+    DBuf<DevFlags> flags;
+    DBuf<DParams> dparams;
+
+    std::vector<int32_t> bonds_h, btype_h;
+    bool have_bonds = false;
+
+    DParams P{};
+    bool pair_set[6] = {false, false, false, false, false, false};
+    double rcut_max[6] = {0, 0, 0, 0, 0, 0};
+    bool bond_set = false;
+    bool params_dirty = true;
+
+    double buffer = 0.4;
+    double rlist = 0.0;
+    int nc[3] = {1, 1, 1};
+    uint32_t cell_cap = 0;
+    uint32_t max_neigh = 0;
+    bool nlist_valid = false;
+
+    bool have_langevin = false;
+    double dt = 0.003, kT = 0.1, gamma = 1.0, gamma_r = 1.0;
+    uint64_t seed = 0;
+
+    uint32_t forces_mask = 0xffffffffu;  // mask of the forces currently in `force`
+    hipGraphExec_t graph = nullptr;
+    uint32_t graph_mask = 0;
+    int graph_steps = 0;
+    uint64_t host_step = 0;
+
+    ~oxdna_ctx() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        pos.release(); vel.release(); quat.release(); angmom.release(); pos_last.release();
+        bd.release(); bd_tq.release();
+        charge.release(); fr0.release(); fr1.release(); fr2.release(); force.release(); torque.release();
+        virial.release(); bpart.release(); binfo.release(); n_neigh.release(); nlist.release();
+        cell_count.release(); cell_members.release(); head.release(); epart.release(); energy.release();
+        flags.release(); dparams.release();
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_steps = 0;
+    }
+};
+
+namespace {
+
+int fail(oxdna_ctx* ctx, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    else g_error = msg;
+    return -1;
+}
+
+template<class F> int guard(oxdna_ctx* ctx, F&& body) {
+    try {
+        if (ctx) {
+            ctx->err.clear();
+            HIP_OK(hipSetDevice(ctx->device));
+        }
+        body();
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(ctx, e.what());
+    }
+}
+
+uint32_t blocks_for(size_t threads) { return (uint32_t)((threads + kBlock - 1) / kBlock); }
+
+// ------------------------------------------------------------------ parameter conversion
+void cvt3(float d[3], const double s[3]) { d[0] = (float)s[0]; d[1] = (float)s[1]; d[2] = (float)s[2]; }
+F4P cvt_f4(const double s[5]) { return F4P{(float)s[0], (float)s[1], (float)s[2], (float)s[3], (float)s[4]}; }
+F5P cvt_f5(const double s[4]) { return F5P{(float)s[0], (float)s[1], (float)s[2], (float)s[3]}; }
+
+template<class T> void require_uniform(const T* tab, int nt, const char* term, T (*clear)(T)) {
+    T ref = clear(tab[0]);
+    for (int k = 1; k < nt * nt; ++k) {
+        T x = clear(tab[k]);
+        if (std::memcmp(&ref, &x, sizeof(T)) != 0)
+            throw Fail(std::string(term) + ": parameters other than the per-type-pair HB epsilon must be the "
+                       "same for every type pair");
+    }
+}
+
+oxdna_excvol_params keep_ev(oxdna_excvol_params p) { return p; }
+oxdna_hbond_params clear_hb(oxdna_hbond_params p) { p.eps = 0; return p; }
+oxdna_crossstacking_params clear_cs(oxdna_crossstacking_params p) { p._pad = 0; return p; }
+oxdna_hbondcross_params clear_hbx(oxdna_hbondcross_params p) { p.eps = 0; p._pad = 0; return p; }
+oxdna_coaxstacking_params clear_cx(oxdna_coaxstacking_params p) { p._pad = 0; return p; }
+oxdna_debye_params keep_dh(oxdna_debye_params p) { return p; }
+
+void set_f1(F1P& d, double a, double r0, double rlow, double rhigh, double rclow, double rchigh, double blow,
+            double bhigh) {
+    d = F1P{(float)a, (float)r0, (float)rlow, (float)rhigh, (float)rclow, (float)rchigh, (float)blow, (float)bhigh};
+}
+void set_f2(F2P& d, double k, double r0, double rc, double rlow, double rhigh, double rclow, double rchigh,
+            double blow, double bhigh) {
+    d = F2P{(float)k, (float)r0, (float)rc, (float)rlow, (float)rhigh, (float)rclow, (float)rchigh, (float)blow,
+            (float)bhigh};
+}
+
+void load_pair_params(DParams& P, int term, int nt, const void* params) {
+    switch (term) {
+    case OXDNA_TERM_EXCVOL: {
+        auto* t = (const oxdna_excvol_params*)params;
+        require_uniform(t, nt, "excluded volume", keep_ev);
+        DExcVol& d = P.ev;
+        d.eps = (float)t[0].eps;
+        cvt3(d.base, t[0].base_site); cvt3(d.back, t[0].back_site);
+        for (int k = 0; k < 4; ++k) {
+            d.sigma[k] = (float)t[0].sigma[k]; d.rstar[k] = (float)t[0].rstar[k];
+            d.b[k] = (float)t[0].b[k]; d.rc[k] = (float)t[0].rc[k];
+        }
+        break;
+    }
+    case OXDNA_TERM_HBOND: {
+        auto* t = (const oxdna_hbond_params*)params;
+        require_uniform(t, nt, "hydrogen bonding", clear_hb);
+        DBaseBase& d = P.hb;
+        cvt3(d.base, t[0].base_site);
+        set_f1(d.f1, t[0].f1_a, t[0].f1_r0, t[0].f1_rlow, t[0].f1_rhigh, t[0].f1_rclow, t[0].f1_rchigh,
+               t[0].f1_blow, t[0].f1_bhigh);
+        d.shift_factor = (float)t[0].shift_factor;
+        for (int k = 0; k < 16; ++k) d.hb_eps[k] = k < nt * nt ? (float)t[k].eps : 0.f;
+        d.hb_t1 = cvt_f4(t[0].t1); d.hb_t23 = cvt_f4(t[0].t23); d.hb_t4 = cvt_f4(t[0].t4); d.hb_t78 = cvt_f4(t[0].t78);
+        break;
+    }
+    case OXDNA_TERM_CROSS: {
+        auto* t = (const oxdna_crossstacking_params*)params;
+        require_uniform(t, nt, "cross-stacking", clear_cs);
+        DBaseBase& d = P.cs;
+        cvt3(d.base, t[0].base_site);
+        set_f2(d.f2, t[0].f2_k, t[0].f2_r0, t[0].f2_rc, t[0].f2_rlow, t[0].f2_rhigh, t[0].f2_rclow, t[0].f2_rchigh,
+               t[0].f2_blow, t[0].f2_bhigh);
+        d.cs_t1 = cvt_f4(t[0].t1); d.cs_t23 = cvt_f4(t[0].t23); d.cs_t4 = cvt_f4(t[0].t4); d.cs_t78 = cvt_f4(t[0].t78);
+        d.cs_t4_enabled = t[0].t4_enabled;
+        break;
+    }
+    case OXDNA_TERM_HBONDCROSS: {
+        auto* t = (const oxdna_hbondcross_params*)params;
+        require_uniform(t, nt, "HB + cross-stacking", clear_hbx);
+        DBaseBase& d = P.hbx;
+        cvt3(d.base, t[0].base_site);
+        set_f1(d.f1, t[0].f1_a, t[0].f1_r0, t[0].f1_rlow, t[0].f1_rhigh, t[0].f1_rclow, t[0].f1_rchigh,
+               t[0].f1_blow, t[0].f1_bhigh);
+        d.shift_factor = (float)t[0].shift_factor;
+        for (int k = 0; k < 16; ++k) d.hb_eps[k] = k < nt * nt ? (float)t[k].eps : 0.f;
+        d.hb_t1 = cvt_f4(t[0].hb_t1); d.hb_t23 = cvt_f4(t[0].hb_t23);
+        d.hb_t4 = cvt_f4(t[0].hb_t4); d.hb_t78 = cvt_f4(t[0].hb_t78);
+        set_f2(d.f2, t[0].f2_k, t[0].f2_r0, t[0].f2_rc, t[0].f2_rlow, t[0].f2_rhigh, t[0].f2_rclow, t[0].f2_rchigh,
+               t[0].f2_blow, t[0].f2_bhigh);
+        d.cs_t1 = cvt_f4(t[0].cs_t1); d.cs_t23 = cvt_f4(t[0].cs_t23);
+        d.cs_t4 = cvt_f4(t[0].cs_t4); d.cs_t78 = cvt_f4(t[0].cs_t78);
+        d.cs_t4_enabled = t[0].cs_t4_enabled;
+        break;
+    }
+    case OXDNA_TERM_COAX: {
+        auto* t = (const oxdna_coaxstacking_params*)params;
+        require_uniform(t, nt, "coaxial stacking", clear_cx);
+        DCoax& d = P.cx;
+        cvt3(d.stack, t[0].stack_site); cvt3(d.backref, t[0].stack_back_ref);
+        d.gamma = (float)t[0].gamma;
+        set_f2(d.f2, t[0].f2_k, t[0].f2_r0, t[0].f2_rc, t[0].f2_rlow, t[0].f2_rhigh, t[0].f2_rclow, t[0].f2_rchigh,
+               t[0].f2_blow, t[0].f2_bhigh);
+        d.t1 = cvt_f4(t[0].t1); d.t4 = cvt_f4(t[0].t4); d.t56 = cvt_f4(t[0].t56);
+        d.phi3 = cvt_f5(t[0].phi3); d.phi4 = cvt_f5(t[0].phi4);
+        d.t1_mode = t[0].t1_mode; d.phi3_enabled = t[0].phi3_enabled; d.rna_coax = t[0].rna_coax;
+        d.t1_sa = (float)t[0].t1_sa; d.t1_sb = (float)t[0].t1_sb;
+        if (d.t1_mode != 0 && d.t1_mode != 1) throw Fail("coaxial stacking: t1_mode must be 0 or 1");
+        break;
+    }
+    case OXDNA_TERM_DEBYE: {
+        auto* t = (const oxdna_debye_params*)params;
+        require_uniform(t, nt, "Debye-Hueckel", keep_dh);
+        DDebye& d = P.dh;
+        cvt3(d.back, t[0].back_site);
+        d.minus_kappa = (float)t[0].minus_kappa; d.prefactor = (float)t[0].prefactor;
+        d.b_smooth = (float)t[0].b_smooth; d.rc = (float)t[0].rc; d.rhigh = (float)t[0].rhigh;
+        break;
+    }
+    default:
+        throw Fail("unknown nonbonded term id");
+    }
+}
+
+void load_bond_params(DBonded& d, const oxdna_bonded_params& s) {
+    std::memset(&d, 0, sizeof(d));
+    d.eps_backbone = s.eps_backbone; d.r0_backbone = s.r0_backbone;
+    d.delta_backbone = s.delta_backbone; d.delta2 = s.delta2;
+    cvt3(d.back, s.back_site); cvt3(d.base, s.base_site);
+    d.excl_eps = (float)s.excl_eps;
+    for (int k = 0; k < 3; ++k) {
+        d.bx_sigma[k] = (float)s.bx_sigma[k]; d.bx_rstar[k] = (float)s.bx_rstar[k];
+        d.bx_b[k] = (float)s.bx_b[k]; d.bx_rc[k] = (float)s.bx_rc[k];
+    }
+    d.stacking_enabled = s.stacking_enabled;
+    d.rna_stacking = s.rna_stacking;
+    cvt3(d.stack, s.stack_site); cvt3(d.backref, s.stack_back_ref);
+    d.gamma = (float)s.st_gamma;
+    set_f1(d.f1, s.st_a, s.st_r0, s.st_rlow, s.st_rhigh, s.st_rclow, s.st_rchigh, s.st_blow, s.st_bhigh);
+    d.shift_factor = (float)s.st_shift_factor;
+    for (int k = 0; k < 16; ++k) d.st_eps[k] = (float)s.st_eps[k];
+    d.t4 = cvt_f4(s.t4); d.t5 = cvt_f4(s.t5); d.p1 = cvt_f5(s.p1);
+    cvt3(d.stack3, s.stack_3_site); cvt3(d.stack5, s.stack_5_site);
+    cvt3(d.bbv3, s.bbvector_3); cvt3(d.bbv5, s.bbvector_5);
+    d.tB1 = cvt_f4(s.tB1); d.tB2 = cvt_f4(s.tB2);
+}
+
+double pair_rmax(const oxdna_ctx* c, uint32_t mask) {
+    double r = 0.0;
+    for (int t = 0; t < 6; ++t)
+        if (((mask >> t) & 1u) && c->pair_set[t]) r = std::max(r, c->rcut_max[t]);
+    return r;
+}
+
+uint32_t effective_mask(const oxdna_ctx* c, uint32_t mask) {
+    uint32_t m = 0;
+    for (int t = 0; t < 6; ++t)
+        if (((mask >> t) & 1u) && c->pair_set[t]) m |= 1u << t;
+    if (((mask >> OXDNA_TERM_BONDED) & 1u) && c->bond_set && c->have_bonds) m |= 1u << OXDNA_TERM_BONDED;
+    return m;
+}
+
+void upload_params(oxdna_ctx* c) {
+    if (!c->params_dirty) return;
+    c->dparams.alloc(1);
+    HIP_OK(hipMemcpyAsync(c->dparams.p, &c->P, sizeof(DParams), hipMemcpyHostToDevice, c->stream));
+    c->params_dirty = false;
+}
+
+// ------------------------------------------------------------------ neighbour list management
+void setup_cells(oxdna_ctx* c) {
+    // neighbour radius: the largest cut-off of every attached nonbonded term + skin
+    double rmax = pair_rmax(c, 0x3f);
+    c->rlist = rmax > 0 ? rmax + c->buffer : 0.0;
+    size_t total = 1;
+    for (int a = 0; a < 3; ++a) {
+        int n = c->rlist > 0 ? (int)std::floor(c->L[a] / c->rlist) : 1;
+        c->nc[a] = std::max(1, std::min(n, 1024));
+        total *= (size_t)c->nc[a];
+    }
+    if (c->cell_cap == 0) {
+        double per_cell = (double)c->N / (double)total;
+        c->cell_cap = (uint32_t)std::max(16.0, std::ceil(per_cell * 4.0 + 8.0));
+    }
+    c->cell_count.alloc(total);
+    c->cell_members.alloc(total * c->cell_cap);
+    hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for(total), 4096)), dim3(kBlock), 0, c->stream,
+                       c->cell_count.p, total);
+    if (c->max_neigh == 0) c->max_neigh = 64;
+    c->nlist.alloc((size_t)c->N * c->max_neigh);
+    std::vector<size_t> h(c->N);
+    for (uint32_t i = 0; i < c->N; ++i) h[i] = (size_t)i * c->max_neigh;
+    c->head.alloc(c->N);
+    HIP_OK(hipMemcpyAsync(c->head.p, h.data(), sizeof(size_t) * c->N, hipMemcpyHostToDevice, c->stream));
+}
+
+size_t ncell_total(const oxdna_ctx* c) { return (size_t)c->nc[0] * c->nc[1] * c->nc[2]; }
+
+CellArgs cell_args(oxdna_ctx* c) {
+    CellArgs a;
+    a.N = c->N;
+    a.pos = c->pos.p;
+    for (int k = 0; k < 3; ++k) { a.Linv[k] = c->Linv[k]; a.nc[k] = c->nc[k]; }
+    a.cap = c->cell_cap;
+    a.count = c->cell_count.p;
+    a.members = c->cell_members.p;
+    a.flags = c->flags.p;
+    return a;
+}
+
+NlistArgs nlist_args(oxdna_ctx* c, int conditional) {
+    NlistArgs a;
+    a.N = c->N;
+    a.pos = c->pos.p;
+    for (int k = 0; k < 3; ++k) { a.L[k] = c->L[k]; a.Linv[k] = c->Linv[k]; a.nc[k] = c->nc[k]; }
+    a.cap = c->cell_cap;
+    a.count = c->cell_count.p;
+    a.members = c->cell_members.p;
+    a.bpart = c->bpart.p;
+    a.rlist2 = c->rlist * c->rlist;
+    a.max_neigh = c->max_neigh;
+    a.n_neigh = c->n_neigh.p;
+    a.nlist = c->nlist.p;
+    a.pos_last = c->pos_last.p;
+    a.flags = c->flags.p;
+    a.conditional = conditional;
+    return a;
+}
+
+void read_flags(oxdna_ctx* c, DevFlags& f) {
+    HIP_OK(hipMemcpyAsync(&f, c->flags.p, sizeof(DevFlags), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+}
+
+void clear_overflow(oxdna_ctx* c) {
+    DevFlags f;
+    read_flags(c, f);
+    f.overflow = 0; f.max_cell = 0; f.max_neigh = 0; f.rebuild = 0;
+    HIP_OK(hipMemcpyAsync(c->flags.p, &f, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
+}
+
+// Unconditional rebuild from the current positions, growing buffers until nothing overflows.
+void build_nlist(oxdna_ctx* c) {
+    if (c->rlist <= 0.0) setup_cells(c);
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        clear_overflow(c);
+        hipLaunchKernelGGL(k_cell_assign, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, cell_args(c));
+        hipLaunchKernelGGL(k_nlist_build, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, nlist_args(c, 0));
+        size_t nct = ncell_total(c);
+        hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for(nct), 4096)), dim3(kBlock), 0, c->stream,
+                           c->cell_count.p, nct);
+        DevFlags f;
+        read_flags(c, f);
+        if (f.overflow == 0) {
+            // keep headroom so that rows can grow between rebuilds during a run
+            uint32_t want = std::max<uint32_t>(16, (uint32_t)std::ceil(f.max_neigh * 1.5) + 8);
+            if (want > c->max_neigh) {
+                c->max_neigh = want;
+                c->drop_graph();
+                setup_cells(c);
+                continue;
+            }
+            uint32_t want_cap = (uint32_t)std::ceil(f.max_cell * 1.5) + 4;
+            if (want_cap > c->cell_cap) {
+                c->cell_cap = want_cap;
+                c->drop_graph();
+                setup_cells(c);
+                continue;
+            }
+            c->nlist_valid = true;
+            return;
+        }
+        if (f.overflow & 1u) c->cell_cap = (uint32_t)std::ceil(f.max_cell * 1.5) + 4;
+        if (f.overflow & 2u) c->max_neigh = (uint32_t)std::ceil(f.max_neigh * 1.5) + 8;
+        c->drop_graph();
+        setup_cells(c);
+    }
+    throw Fail("neighbour list could not be sized");
+}
+
+void ensure_ready(oxdna_ctx* c) {
+    if (c->N == 0) throw Fail("no particles: call oxdna_set_state first");
+    if (c->params_dirty) {
+        c->drop_graph();
+        setup_cells(c);
+        c->nlist_valid = false;
+    }
+    upload_params(c);
+}
+
+ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, bool zero_cells) {
+    ForceArgs a;
+    a.N = c->N;
+    a.pos = c->pos.p;
+    a.fr0 = c->fr0.p; a.fr1 = c->fr1.p; a.fr2 = c->fr2.p;
+    a.n_neigh = c->n_neigh.p;
+    a.nlist = c->nlist.p;
+    a.head = c->head.p;
+    a.bpart = c->bpart.p;
+    a.binfo = c->binfo.p;
+    a.P = c->dparams.p;
+    for (int k = 0; k < 3; ++k) { a.L[k] = c->L[k]; a.Linv[k] = c->Linv[k]; }
+    a.mask = mask;
+    a.force = c->force.p;
+    a.torque = c->torque.p;
+    a.virial = c->virial.p;
+    a.vpitch = c->N;
+    a.epart = c->epart.p;
+    a.accumulate = accumulate;
+    a.cell_count = zero_cells ? c->cell_count.p : nullptr;
+    a.ncell_total = (uint32_t)ncell_total(c);
+    a.split = nullptr;
+    a.counts = nullptr;
+    return a;
+}
+
+uint32_t force_blocks(const oxdna_ctx* c) { return blocks_for((size_t)c->N * kTPP); }
+
+void launch_forces(oxdna_ctx* c, uint32_t mask, int accumulate, bool want_e, bool want_v, bool zero_cells) {
+    ForceArgs a = force_args(c, mask, accumulate, zero_cells);
+    dim3 grid(force_blocks(c)), block(kBlock);
+    if (want_e && want_v) hipLaunchKernelGGL((k_forces<kTPP, true, true>), grid, block, 0, c->stream, a);
+    else if (want_e) hipLaunchKernelGGL((k_forces<kTPP, true, false>), grid, block, 0, c->stream, a);
+    else if (want_v) hipLaunchKernelGGL((k_forces<kTPP, false, true>), grid, block, 0, c->stream, a);
+    else hipLaunchKernelGGL((k_forces<kTPP, false, false>), grid, block, 0, c->stream, a);
+    if (want_e)
+        hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(256), 0, c->stream, grid.x, c->epart.p, c->energy.p,
+                           accumulate);
+}
+
+void compute(oxdna_ctx* c, uint32_t mask, int fused) {
+    ensure_ready(c);
+    hipLaunchKernelGGL(k_frames, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->quat.p, c->charge.p,
+                       c->pos.p, c->fr0.p, c->fr1.p, c->fr2.p);
+    uint32_t m = effective_mask(c, mask);
+    if (m & 0x3fu) build_nlist(c);
+    if (fused || m == 0) {
+        launch_forces(c, m, 0, true, true, false);
+    } else {
+        int first = 1;
+        for (int t = 0; t < OXDNA_N_TERMS; ++t) {
+            if (!((m >> t) & 1u)) continue;
+            launch_forces(c, 1u << t, first ? 0 : 1, true, true, false);
+            first = 0;
+        }
+    }
+    c->forces_mask = m;
+}
+
+StepArgs step_args(oxdna_ctx* c) {
+    StepArgs a;
+    a.N = c->N;
+    a.bd = c->bd.p; a.bd_tq = c->bd_tq.p;
+    a.pos = c->pos.p; a.vel = c->vel.p; a.quat = c->quat.p; a.angmom = c->angmom.p;
+    a.force = c->force.p; a.torque = c->torque.p; a.charge = c->charge.p;
+    a.fr0 = c->fr0.p; a.fr1 = c->fr1.p; a.fr2 = c->fr2.p;
+    a.pos_last = c->pos_last.p;
+    a.dt = c->dt; a.kT = c->kT; a.gamma = c->gamma; a.gamma_r = c->gamma_r; a.mass = 1.0; a.inertia = 1.0;
+    for (int k = 0; k < 3; ++k) { a.L[k] = c->L[k]; a.Linv[k] = c->Linv[k]; }
+    a.skin_half2 = 0.25 * c->buffer * c->buffer;
+    a.seed = c->seed;
+    a.flags = c->flags.p;
+    a.cells = cell_args(c);
+    return a;
+}
+
+void enqueue_step(oxdna_ctx* c, uint32_t mask) {
+    StepArgs s = step_args(c);
+    hipLaunchKernelGGL(k_step1, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, s);
+    if (mask & 0x3fu)
+        hipLaunchKernelGGL(k_nlist_build, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, nlist_args(c, 1));
+    launch_forces(c, mask, 0, false, false, true);
+    hipLaunchKernelGGL(k_step2, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, s);
+}
+
+constexpr int kGraphSteps = 16;
+
+void build_graph(oxdna_ctx* c, uint32_t mask) {
+    c->drop_graph();
+    hipGraph_t g;
+    HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < kGraphSteps; ++s) enqueue_step(c, mask);
+    HIP_OK(hipStreamEndCapture(c->stream, &g));
+    HIP_OK(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+    HIP_OK(hipGraphDestroy(g));
+    c->graph_mask = mask;
+    c->graph_steps = kGraphSteps;
+}
+
+struct Saved {
+    DBuf<double4> pos, vel, quat, angmom, pos_last, bd, bd_tq;
+    DBuf<float4> force, torque;
+    ~Saved() { pos.release(); vel.release(); quat.release(); angmom.release(); pos_last.release();
+               bd.release(); bd_tq.release();
+               force.release(); torque.release(); }
+};
+
+void save_state(oxdna_ctx* c, Saved& s) {
+    auto cp = [&](auto& dst, auto& src) {
+        dst.alloc(c->N);
+        HIP_OK(hipMemcpyAsync(dst.p, src.p, sizeof(*src.p) * c->N, hipMemcpyDeviceToDevice, c->stream));
+    };
+    cp(s.pos, c->pos); cp(s.vel, c->vel); cp(s.quat, c->quat); cp(s.angmom, c->angmom);
+    cp(s.pos_last, c->pos_last); cp(s.force, c->force); cp(s.torque, c->torque);
+    cp(s.bd, c->bd); cp(s.bd_tq, c->bd_tq);
+}
+void restore_state(oxdna_ctx* c, Saved& s) {
+    auto cp = [&](auto& dst, auto& src) {
+        HIP_OK(hipMemcpyAsync(dst.p, src.p, sizeof(*src.p) * c->N, hipMemcpyDeviceToDevice, c->stream));
+    };
+    cp(c->pos, s.pos); cp(c->vel, s.vel); cp(c->quat, s.quat); cp(c->angmom, s.angmom);
+    cp(c->pos_last, s.pos_last); cp(c->force, s.force); cp(c->torque, s.torque);
+    cp(c->bd, s.bd); cp(c->bd_tq, s.bd_tq);
+}
+
+void run(oxdna_ctx* c, uint32_t nsteps, uint32_t mask, int use_graph) {
+    if (!c->have_langevin) throw Fail("no integrator: call oxdna_set_langevin first");
+    ensure_ready(c);
+    uint32_t m = effective_mask(c, mask);
+    if (c->forces_mask != m || !c->nlist_valid) compute(c, m, 1);
+    const uint32_t chunk_max = 4096;
+    uint32_t done = 0;
+    Saved saved;
+    while (done < nsteps) {
+        uint32_t chunk = std::min(chunk_max, nsteps - done);
+        save_state(c, saved);
+        DevFlags f0;
+        read_flags(c, f0);
+        unsigned long long step0 = f0.step;
+        for (int attempt = 0;; ++attempt) {
+            uint32_t left = chunk;
+            if (use_graph) {
+                if (!c->graph || c->graph_mask != m) build_graph(c, m);
+                while (left >= (uint32_t)c->graph_steps) {
+                    HIP_OK(hipGraphLaunch(c->graph, c->stream));
+                    left -= c->graph_steps;
+                }
+            }
+            while (left > 0) { enqueue_step(c, m); --left; }
+            DevFlags f;
+            read_flags(c, f);
+            if (f.overflow == 0) break;
+            if (attempt >= 4) throw Fail("neighbour list kept overflowing during the run");
+            // grow, restore the chunk's starting state and run it again
+            restore_state(c, saved);
+            f.overflow = 0; f.step = step0; f.rebuild = 0;
+            HIP_OK(hipMemcpyAsync(c->flags.p, &f, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
+            c->max_neigh = std::max(c->max_neigh, (uint32_t)std::ceil(f.max_neigh * 1.5) + 8);
+            c->cell_cap = std::max(c->cell_cap, (uint32_t)std::ceil(f.max_cell * 1.5) + 4);
+            c->drop_graph();
+            setup_cells(c);
+            HIP_OK(hipStreamSynchronize(c->stream));
+            build_nlist(c);
+        }
+        done += chunk;
+    }
+    c->host_step += nsteps;
+}
+
+}  // namespace
+
+// =====================================================================================
+// C ABI
+// =====================================================================================
+extern "C" {
+
+const char* oxdna_last_error(const oxdna_ctx* ctx) { return ctx ? ctx->err.c_str() : g_error.c_str(); }
+
+int oxdna_abi_version(void) { return OXDNA_ABI_VERSION; }
+
+int oxdna_device_name(int device, char* buf, size_t buflen) {
+    return guard(nullptr, [&] {
+        hipDeviceProp_t prop;
+        HIP_OK(hipGetDeviceProperties(&prop, device));
+        std::snprintf(buf, buflen, "%s", prop.gcnArchName);
+    });
+}
+
+int oxdna_ctx_create(int device, void* stream, oxdna_ctx** out) {
+    if (!out) return fail(nullptr, "oxdna_ctx_create: out is NULL");
+    *out = nullptr;
+    oxdna_ctx* c = new oxdna_ctx();
+    c->device = device;
+    int rc = guard(nullptr, [&] {
+        int count = 0;
+        HIP_OK(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) throw Fail("oxdna_ctx_create: no such HIP device");
+        HIP_OK(hipSetDevice(device));
+        if (stream) c->stream = (hipStream_t)stream;
+        else {
+            HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            c->own_stream = true;
+        }
+        c->flags.alloc(1);
+        DevFlags f{};
+        HIP_OK(hipMemcpy(c->flags.p, &f, sizeof(f), hipMemcpyHostToDevice));
+        c->energy.alloc(8);
+        HIP_OK(hipMemset(c->energy.p, 0, 8 * sizeof(double)));
+    });
+    if (rc != 0) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+int oxdna_ctx_destroy(oxdna_ctx* ctx) {
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    delete ctx;
+    return 0;
+}
+
+int oxdna_set_state(oxdna_ctx* c, unsigned int n, const double* pos, const double* quat, const int32_t* type,
+                    const double* charge, const oxdna_box* box) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (n == 0) throw Fail("oxdna_set_state: n must be > 0");
+        if (!pos || !quat || !type || !box) throw Fail("oxdna_set_state: NULL array");
+        for (int a = 0; a < 3; ++a)
+            if (!(box->L[a] > 0.0)) throw Fail("oxdna_set_state: box lengths must be positive");
+        bool resized = n != c->N;
+        c->N = n;
+        for (int a = 0; a < 3; ++a) { c->L[a] = box->L[a]; c->Linv[a] = 1.0 / box->L[a]; }
+        std::vector<double4> p(n), q(n);
+        std::vector<double> ch(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            if (type[i] < 0 || type[i] >= 4) throw Fail("oxdna_set_state: type ids must be in [0, 4)");
+            double w;
+            uint64_t bits = (uint64_t)(uint32_t)type[i];
+            std::memcpy(&w, &bits, sizeof(w));
+            p[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], w);
+            q[i] = make_double4(quat[4 * i], quat[4 * i + 1], quat[4 * i + 2], quat[4 * i + 3]);
+            ch[i] = charge ? charge[i] : 1.0;
+        }
+        if (resized) {
+            c->pos.release(); c->quat.release(); c->vel.release(); c->angmom.release(); c->pos_last.release();
+            c->bd.release(); c->bd_tq.release();
+            c->charge.release(); c->fr0.release(); c->fr1.release(); c->fr2.release(); c->force.release();
+            c->torque.release(); c->virial.release(); c->n_neigh.release(); c->nlist.release(); c->head.release();
+            c->epart.release();
+            c->have_bonds = false;
+        }
+        c->pos.alloc(n); c->quat.alloc(n); c->vel.alloc(n); c->angmom.alloc(n); c->pos_last.alloc(n);
+        c->bd.alloc(n); c->bd_tq.alloc(n);
+        c->charge.alloc(n); c->fr0.alloc(n); c->fr1.alloc(n); c->fr2.alloc(n);
+        c->force.alloc(n); c->torque.alloc(n); c->virial.alloc(6 * (size_t)n); c->n_neigh.alloc(n);
+        c->epart.alloc((size_t)blocks_for((size_t)n * kTPP) * 8);
+        HIP_OK(hipMemcpyAsync(c->pos.p, p.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(c->pos_last.p, p.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(c->quat.p, q.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(c->charge.p, ch.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        if (resized) {
+            HIP_OK(hipMemsetAsync(c->vel.p, 0, sizeof(double4) * n, c->stream));
+            HIP_OK(hipMemsetAsync(c->angmom.p, 0, sizeof(double4) * n, c->stream));
+            HIP_OK(hipMemsetAsync(c->bd.p, 0, sizeof(double4) * n, c->stream));
+            HIP_OK(hipMemsetAsync(c->bd_tq.p, 0, sizeof(double4) * n, c->stream));
+            std::vector<int> none(2 * (size_t)n, -1);
+            c->bpart.alloc(2 * (size_t)n); c->binfo.alloc(2 * (size_t)n);
+            HIP_OK(hipMemcpyAsync(c->bpart.p, none.data(), sizeof(int) * 2 * n, hipMemcpyHostToDevice, c->stream));
+            HIP_OK(hipMemcpyAsync(c->binfo.p, none.data(), sizeof(int) * 2 * n, hipMemcpyHostToDevice, c->stream));
+            c->cell_cap = 0;
+            c->max_neigh = 0;
+            c->params_dirty = true;
+        }
+        HIP_OK(hipStreamSynchronize(c->stream));
+        c->drop_graph();
+        c->nlist_valid = false;
+        c->forces_mask = 0xffffffffu;
+    });
+}
+
+int oxdna_set_momenta(oxdna_ctx* c, const double* vel, const double* angmom) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (c->N == 0) throw Fail("oxdna_set_momenta: set the state first");
+        std::vector<double4> v(c->N), p(c->N);
+        for (uint32_t i = 0; i < c->N; ++i) {
+            v[i] = vel ? make_double4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0) : make_double4(0, 0, 0, 0);
+            p[i] = angmom ? make_double4(angmom[4 * i], angmom[4 * i + 1], angmom[4 * i + 2], angmom[4 * i + 3])
+                          : make_double4(0, 0, 0, 0);
+        }
+        HIP_OK(hipMemcpyAsync(c->vel.p, v.data(), sizeof(double4) * c->N, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(c->angmom.p, p.data(), sizeof(double4) * c->N, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int oxdna_set_bonds(oxdna_ctx* c, unsigned int nb, const int32_t* ab, const int32_t* btype) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (c->N == 0) throw Fail("oxdna_set_bonds: set the state first");
+        std::vector<int> part(2 * (size_t)c->N, -1), info(2 * (size_t)c->N, -1);
+        for (uint32_t b = 0; b < nb; ++b) {
+            int A = ab[2 * b], B = ab[2 * b + 1];
+            int t = btype ? btype[b] : 0;
+            if (A < 0 || B < 0 || A >= (int)c->N || B >= (int)c->N || A == B)
+                throw Fail("oxdna_set_bonds: bond " + std::to_string(b) + " has an invalid particle index");
+            if (t < 0 || t >= MAX_BOND_TYPES) throw Fail("oxdna_set_bonds: bond type out of range");
+            int ends[2] = {A, B};
+            for (int r = 0; r < 2; ++r) {
+                int i = ends[r];
+                int slot = part[2 * i] < 0 ? 0 : (part[2 * i + 1] < 0 ? 1 : -1);
+                if (slot < 0)
+                    throw Fail("oxdna_set_bonds: particle " + std::to_string(i) +
+                               " has more than two backbone bonds");
+                part[2 * i + slot] = ends[1 - r];
+                info[2 * i + slot] = r | (t << 1);
+            }
+        }
+        c->bpart.alloc(2 * (size_t)c->N);
+        c->binfo.alloc(2 * (size_t)c->N);
+        HIP_OK(hipMemcpyAsync(c->bpart.p, part.data(), sizeof(int) * 2 * c->N, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(c->binfo.p, info.data(), sizeof(int) * 2 * c->N, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        c->have_bonds = nb > 0;
+        c->nlist_valid = false;
+        c->forces_mask = 0xffffffffu;
+        c->drop_graph();
+    });
+}
+
+int oxdna_set_pair_params(oxdna_ctx* c, int term, int ntypes, const void* params, const double* rcut) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (term < 0 || term >= OXDNA_TERM_BONDED) throw Fail("oxdna_set_pair_params: not a nonbonded term id");
+        if (!params) {
+            c->pair_set[term] = false;
+            c->rcut_max[term] = 0.0;
+        } else {
+            if (ntypes < 1 || ntypes > 4) throw Fail("oxdna_set_pair_params: ntypes must be 1..4");
+            if (!rcut) throw Fail("oxdna_set_pair_params: rcut table is NULL");
+            load_pair_params(c->P, term, ntypes, params);
+            double rmax = 0.0;
+            for (int k = 0; k < 16; ++k) {
+                double r = k < ntypes * ntypes ? rcut[k] : 0.0;
+                if (r < 0) throw Fail("oxdna_set_pair_params: negative cut-off");
+                c->P.rcutsq[term][k] = (float)(r * r);
+                rmax = std::max(rmax, r);
+            }
+            c->P.ntypes = ntypes;
+            c->pair_set[term] = true;
+            c->rcut_max[term] = rmax;
+        }
+        c->P.mask = 0;
+        for (int t = 0; t < 6; ++t) c->P.mask |= c->pair_set[t] ? (1u << t) : 0u;
+        c->params_dirty = true;
+        c->forces_mask = 0xffffffffu;
+    });
+}
+
+int oxdna_set_bond_params(oxdna_ctx* c, unsigned int nbt, const oxdna_bonded_params* params) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (!params || nbt == 0) {
+            c->bond_set = false;
+        } else {
+            if (nbt > MAX_BOND_TYPES) throw Fail("oxdna_set_bond_params: at most 4 bond types");
+            for (uint32_t k = 0; k < nbt; ++k) {
+                if (!(params[k].delta_backbone > 0.0)) throw Fail("oxdna_set_bond_params: delta must be > 0");
+                load_bond_params(c->P.bond[k], params[k]);
+            }
+            c->P.n_bond_types = (int)nbt;
+            c->bond_set = true;
+        }
+        c->params_dirty = true;
+        c->forces_mask = 0xffffffffu;
+    });
+}
+
+int oxdna_set_nlist(oxdna_ctx* c, double buffer) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (!(buffer >= 0.0)) throw Fail("oxdna_set_nlist: buffer must be >= 0");
+        c->buffer = buffer;
+        c->params_dirty = true;
+        c->nlist_valid = false;
+    });
+}
+
+int oxdna_compute(oxdna_ctx* c, unsigned int mask, int fused) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] { compute(c, mask, fused); });
+}
+
+int oxdna_get_term_energies(oxdna_ctx* c, double out[8]) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        HIP_OK(hipMemcpyAsync(out, c->energy.p, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int oxdna_get_forces(oxdna_ctx* c, double* force4, double* torque3, double* virial6) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        uint32_t n = c->N;
+        std::vector<float4> f(n), t(n);
+        std::vector<float> v(6 * (size_t)n);
+        HIP_OK(hipMemcpyAsync(f.data(), c->force.p, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(t.data(), c->torque.p, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(v.data(), c->virial.p, sizeof(float) * 6 * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        for (uint32_t i = 0; i < n; ++i) {
+            if (force4) { force4[4 * i] = f[i].x; force4[4 * i + 1] = f[i].y; force4[4 * i + 2] = f[i].z; force4[4 * i + 3] = f[i].w; }
+            if (torque3) { torque3[3 * i] = t[i].x; torque3[3 * i + 1] = t[i].y; torque3[3 * i + 2] = t[i].z; }
+            if (virial6)
+                for (int s = 0; s < 6; ++s) virial6[6 * i + s] = v[(size_t)s * n + i];
+        }
+    });
+}
+
+int oxdna_get_nlist_pairs(oxdna_ctx* c, int32_t* pairs, size_t max_pairs, size_t* npairs) {
+    if (!c) return fail(nullptr, "NULL context");
+    int rc = 0;
+    int g = guard(c, [&] {
+        if (!c->nlist_valid) {
+            ensure_ready(c);
+            build_nlist(c);
+        }
+        uint32_t n = c->N;
+        std::vector<uint32_t> nn(n), rows((size_t)n * c->max_neigh);
+        HIP_OK(hipMemcpyAsync(nn.data(), c->n_neigh.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(rows.data(), c->nlist.p, sizeof(uint32_t) * rows.size(), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        size_t cnt = 0;
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t k = 0; k < nn[i]; ++k) {
+                uint32_t j = rows[(size_t)i * c->max_neigh + k];
+                if (j > i) {
+                    if (cnt < max_pairs) { pairs[2 * cnt] = (int32_t)i; pairs[2 * cnt + 1] = (int32_t)j; }
+                    ++cnt;
+                }
+            }
+        if (npairs) *npairs = cnt;
+        if (cnt > max_pairs) rc = fail(c, "oxdna_get_nlist_pairs: buffer too small");
+    });
+    return g != 0 ? g : rc;
+}
+
+int oxdna_set_langevin(oxdna_ctx* c, double dt, double kT, double gamma, double gamma_r, uint64_t seed) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (!(dt > 0.0) || !(kT >= 0.0) || !(gamma >= 0.0) || !(gamma_r >= 0.0))
+            throw Fail("oxdna_set_langevin: need dt > 0, kT >= 0, gamma >= 0, gamma_r >= 0");
+        c->dt = dt; c->kT = kT; c->gamma = gamma; c->gamma_r = gamma_r; c->seed = seed;
+        c->have_langevin = true;
+        c->drop_graph();
+    });
+}
+
+int oxdna_run(oxdna_ctx* c, unsigned int nsteps, unsigned int mask, int use_graph) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (nsteps == 0) return;
+        run(c, nsteps, mask, use_graph);
+    });
+}
+
+int oxdna_get_state(oxdna_ctx* c, double* pos, double* quat, double* vel, double* angmom) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        uint32_t n = c->N;
+        std::vector<double4> p(n), q(n), v(n), a(n);
+        HIP_OK(hipMemcpyAsync(p.data(), c->pos.p, sizeof(double4) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(q.data(), c->quat.p, sizeof(double4) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(v.data(), c->vel.p, sizeof(double4) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(a.data(), c->angmom.p, sizeof(double4) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        for (uint32_t i = 0; i < n; ++i) {
+            if (pos) { pos[3 * i] = p[i].x; pos[3 * i + 1] = p[i].y; pos[3 * i + 2] = p[i].z; }
+            if (quat) { quat[4 * i] = q[i].x; quat[4 * i + 1] = q[i].y; quat[4 * i + 2] = q[i].z; quat[4 * i + 3] = q[i].w; }
+            if (vel) { vel[3 * i] = v[i].x; vel[3 * i + 1] = v[i].y; vel[3 * i + 2] = v[i].z; }
+            if (angmom) { angmom[4 * i] = a[i].x; angmom[4 * i + 1] = a[i].y; angmom[4 * i + 2] = a[i].z; angmom[4 * i + 3] = a[i].w; }
+        }
+    });
+}
+
+
+int oxdna_time_forces(oxdna_ctx* c, unsigned int mask, unsigned int nrep, double* avg_ms) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (nrep == 0) throw Fail("oxdna_time_forces: nrep must be > 0");
+        ensure_ready(c);
+        uint32_t m = effective_mask(c, mask);
+        if (c->forces_mask != m || !c->nlist_valid) compute(c, m, 1);
+        std::vector<hipEvent_t> ev(2 * nrep);
+        for (auto& x : ev) HIP_OK(hipEventCreate(&x));
+        for (uint32_t r = 0; r < nrep; ++r) {
+            HIP_OK(hipEventRecord(ev[2 * r], c->stream));
+            launch_forces(c, m, 0, false, false, false);
+            HIP_OK(hipEventRecord(ev[2 * r + 1], c->stream));
+        }
+        HIP_OK(hipStreamSynchronize(c->stream));
+        double tot = 0.0;
+        for (uint32_t r = 0; r < nrep; ++r) {
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]));
+            tot += ms;
+        }
+        for (auto& x : ev) (void)hipEventDestroy(x);
+        *avg_ms = tot / nrep;
+    });
+}
+
+int oxdna_count_evaluations(oxdna_ctx* c, unsigned int mask, uint64_t out[14]) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        ensure_ready(c);
+        uint32_t m = effective_mask(c, mask);
+        if (c->forces_mask != m || !c->nlist_valid) compute(c, m, 1);
+        DBuf<unsigned long long> d;
+        d.alloc(14);
+        HIP_OK(hipMemsetAsync(d.p, 0, 14 * sizeof(unsigned long long), c->stream));
+        ForceArgs a = force_args(c, m, 0, false);
+        a.counts = d.p;
+        hipLaunchKernelGGL((k_forces<kTPP, false, false, true>), dim3(force_blocks(c)), dim3(kBlock), 0, c->stream, a);
+        unsigned long long h[14];
+        HIP_OK(hipMemcpyAsync(h, d.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        d.release();
+        for (int k = 0; k < 14; ++k) out[k] = h[k];
+    });
+}
+
+int oxdna_synchronize(oxdna_ctx* c) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] { HIP_OK(hipStreamSynchronize(c->stream)); });
+}
+
+int oxdna_get_stats(oxdna_ctx* c, uint64_t* n_builds, uint32_t* max_neigh, uint64_t* step) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        DevFlags f;
+        read_flags(c, f);
+        if (n_builds) *n_builds = f.builds;
+        if (max_neigh) *max_neigh = c->max_neigh;
+        if (step) *step = f.step;
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ stateless kernel drivers
+namespace {
+// Library-owned device scratch of the stateless drivers, one set per device (grown on
+// demand, kept until the process ends). Calls that overlap on one device race on it.
+struct DriverScratch {
+    DBuf<float4> fr0, fr1, fr2;
+    DBuf<DParams> P;
+    DBuf<double> epart;
+    DBuf<int> bpart, binfo;
+    DBuf<uint32_t> err, zeros;
+    DBuf<size_t> zhead;
+    std::vector<DParams> staged = std::vector<DParams>(8);
+    int ring = 0;
+};
+DriverScratch g_scratch[64];
+
+DriverScratch& scratch_for(int& dev) {
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw Fail("device index out of range");
+    return g_scratch[dev];
+}
+
+const DParams* stage_params(DriverScratch& s, const DParams& P, hipStream_t st) {
+    s.P.alloc(1);
+    DParams& h = s.staged[s.ring];
+    s.ring = (s.ring + 1) % (int)s.staged.size();
+    h = P;
+    HIP_OK(hipMemcpyAsync(s.P.p, &h, sizeof(DParams), hipMemcpyHostToDevice, st));
+    return s.P.p;
+}
+
+void driver_frames(DriverScratch& s, uint32_t N, const double* d_pos, const double* d_orient, const double* d_charge,
+                   hipStream_t st) {
+    s.fr0.alloc(N); s.fr1.alloc(N); s.fr2.alloc(N);
+    hipLaunchKernelGGL(k_frames, dim3(blocks_for(N)), dim3(kBlock), 0, st, N, (const double4*)d_orient, d_charge,
+                       (const double4*)d_pos, s.fr0.p, s.fr1.p, s.fr2.p);
+}
+
+__global__ void k_bond_table(uint32_t N, const uint32_t* blist, const uint32_t* bpos, uint32_t pitch,
+                             const uint32_t* n_bonds, int* bpart, int* binfo, uint32_t* err) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    uint32_t nb = n_bonds[i];
+    if (nb > 2) atomicOr(err, 1u);
+    for (uint32_t b = 0; b < 2; ++b) {
+        if (b < nb) {
+            size_t k = (size_t)b * pitch + i;
+            bpart[2 * i + b] = (int)blist[2 * k];
+            binfo[2 * i + b] = (int)((bpos[k] & 1u) | (blist[2 * k + 1] << 1));
+        } else {
+            bpart[2 * i + b] = -1;
+            binfo[2 * i + b] = -1;
+        }
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int oxdna_pair_forces(int term, float* d_force, float* d_torque, float* d_virial, size_t virial_pitch,
+                      unsigned int N, const double* d_pos, const double* d_orientation, const double* d_charge,
+                      const oxdna_box* box, const unsigned int* d_n_neigh, const unsigned int* d_nlist,
+                      const size_t* d_head_list, int ntypes, const void* params, const double* rcutsq,
+                      double* d_term_energy, void* stream) {
+    return guard(nullptr, [&] {
+        if (term < 0 || term >= OXDNA_TERM_BONDED) throw Fail("oxdna_pair_forces: not a nonbonded term id");
+        if (ntypes < 1 || ntypes > 4) throw Fail("oxdna_pair_forces: ntypes must be 1..4");
+        if (!params || !rcutsq || !box) throw Fail("oxdna_pair_forces: NULL parameter table, cut-offs or box");
+        if (N == 0) return;
+        if (!d_force || !d_torque || !d_pos || !d_orientation || !d_n_neigh || !d_nlist || !d_head_list)
+            throw Fail("oxdna_pair_forces: NULL device array");
+        if (term == OXDNA_TERM_DEBYE && !d_charge) throw Fail("oxdna_pair_forces: Debye-Hueckel needs charges");
+        hipStream_t st = (hipStream_t)stream;
+        int dev;
+        DriverScratch& s = scratch_for(dev);
+        DParams P{};
+        load_pair_params(P, term, ntypes, params);
+        for (int k = 0; k < ntypes * ntypes; ++k) P.rcutsq[term][k] = (float)rcutsq[k];
+        P.ntypes = ntypes;
+        P.mask = 1u << term;
+        const DParams* dP = stage_params(s, P, st);
+        driver_frames(s, N, d_pos, d_orientation, d_charge, st);
+        uint32_t blocks = blocks_for((size_t)N * kTPP);
+        s.epart.alloc((size_t)blocks * 8);
+        ForceArgs a{};
+        a.N = N;
+        a.pos = (const double4*)d_pos;
+        a.fr0 = s.fr0.p; a.fr1 = s.fr1.p; a.fr2 = s.fr2.p;
+        a.n_neigh = d_n_neigh; a.nlist = d_nlist; a.head = d_head_list;
+        a.P = dP;
+        for (int k = 0; k < 3; ++k) { a.L[k] = box->L[k]; a.Linv[k] = 1.0 / box->L[k]; }
+        a.mask = 1u << term;
+        a.force = (float4*)d_force; a.torque = (float4*)d_torque;
+        a.virial = d_virial; a.vpitch = virial_pitch;
+        a.epart = s.epart.p;
+        bool we = d_term_energy != nullptr, wv = d_virial != nullptr;
+        dim3 g(blocks), b(kBlock);
+        if (we && wv) hipLaunchKernelGGL((k_forces<kTPP, true, true>), g, b, 0, st, a);
+        else if (we) hipLaunchKernelGGL((k_forces<kTPP, true, false>), g, b, 0, st, a);
+        else if (wv) hipLaunchKernelGGL((k_forces<kTPP, false, true>), g, b, 0, st, a);
+        else hipLaunchKernelGGL((k_forces<kTPP, false, false>), g, b, 0, st, a);
+        if (we) hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(256), 0, st, blocks, s.epart.p, d_term_energy, 0);
+        HIP_OK(hipGetLastError());
+    });
+}
+
+int oxdna_bonded_forces(float* d_force, float* d_torque, float* d_virial, size_t virial_pitch,
+                        double* d_split_energy, unsigned int N, const double* d_pos, const double* d_orientation,
+                        const oxdna_box* box, const unsigned int* d_blist, const unsigned int* d_bpos,
+                        unsigned int pitch, const unsigned int* d_n_bonds, const oxdna_bonded_params* params,
+                        unsigned int n_bond_types, void* stream) {
+    return guard(nullptr, [&] {
+        if (!params || n_bond_types == 0 || n_bond_types > MAX_BOND_TYPES)
+            throw Fail("oxdna_bonded_forces: need 1..4 bond parameter sets");
+        if (!box) throw Fail("oxdna_bonded_forces: NULL box");
+        if (N == 0) return;
+        if (!d_force || !d_torque || !d_pos || !d_orientation || !d_blist || !d_bpos || !d_n_bonds)
+            throw Fail("oxdna_bonded_forces: NULL device array");
+        hipStream_t st = (hipStream_t)stream;
+        int dev;
+        DriverScratch& s = scratch_for(dev);
+        DParams P{};
+        for (uint32_t k = 0; k < n_bond_types; ++k) load_bond_params(P.bond[k], params[k]);
+        P.n_bond_types = (int)n_bond_types;
+        P.ntypes = 4;
+        P.mask = 1u << OXDNA_TERM_BONDED;
+        const DParams* dP = stage_params(s, P, st);
+        driver_frames(s, N, d_pos, d_orientation, nullptr, st);
+        s.bpart.alloc(2 * (size_t)N); s.binfo.alloc(2 * (size_t)N); s.err.alloc(1);
+        HIP_OK(hipMemsetAsync(s.err.p, 0, sizeof(uint32_t), st));
+        hipLaunchKernelGGL(k_bond_table, dim3(blocks_for(N)), dim3(kBlock), 0, st, N, d_blist, d_bpos, pitch,
+                           d_n_bonds, s.bpart.p, s.binfo.p, s.err.p);
+        ForceArgs a{};
+        a.N = N;
+        a.pos = (const double4*)d_pos;
+        a.fr0 = s.fr0.p; a.fr1 = s.fr1.p; a.fr2 = s.fr2.p;
+        a.bpart = s.bpart.p; a.binfo = s.binfo.p;
+        a.P = dP;
+        for (int k = 0; k < 3; ++k) { a.L[k] = box->L[k]; a.Linv[k] = 1.0 / box->L[k]; }
+        a.mask = 1u << OXDNA_TERM_BONDED;
+        a.force = (float4*)d_force; a.torque = (float4*)d_torque;
+        a.virial = d_virial; a.vpitch = virial_pitch;
+        a.split = (double4*)d_split_energy;
+        // no nonbonded neighbours: every row is empty
+        s.zeros.alloc(N);
+        s.zhead.alloc(N);
+        hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for(N), 4096)), dim3(kBlock), 0, st, s.zeros.p,
+                           (size_t)N);
+        HIP_OK(hipMemsetAsync(s.zhead.p, 0, sizeof(size_t) * N, st));
+        a.n_neigh = s.zeros.p;
+        a.nlist = s.zeros.p;
+        a.head = s.zhead.p;
+        s.epart.alloc((size_t)blocks_for((size_t)N * kTPP) * 8);
+        a.epart = s.epart.p;
+        dim3 g(blocks_for((size_t)N * kTPP)), b(kBlock);
+        if (d_virial) hipLaunchKernelGGL((k_forces<kTPP, false, true>), g, b, 0, st, a);
+        else hipLaunchKernelGGL((k_forces<kTPP, false, false>), g, b, 0, st, a);
+        uint32_t herr = 0;
+        HIP_OK(hipMemcpyAsync(&herr, s.err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        if (herr) throw Fail("oxdna_bonded_forces: a particle has more than two backbone bonds");
+    });
+}
+
+}  // extern "C"
