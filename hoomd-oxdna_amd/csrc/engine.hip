@@ -8,11 +8,11 @@
 //   force, torque float4[N] (force.w = half energy of i), virial float[6][N]
 //   nlist  uint32[N][max_neigh] rows sorted by index, n_neigh uint32[N]
 //   cells  uint32[ncell] counts + uint32[ncell][cell_cap] members
-// One step = 4 kernels on one stream (captured as a HIP graph):
+// One step = 3 kernels on one stream (captured as a HIP graph):
 //   step1 (half kick, drift, NO_SQUISH rotation, frames, cell binning, skin test)
-//   -> nlist build (exits at once unless the skin test fired)
-//   -> fused forces (every selected term + bonded in one neighbour traversal)
-//   -> step2 (Langevin half kick, translational + rotational).
+//   -> fused forces (neighbour rows rebuilt in-kernel when the skin test fired; every
+//      selected nonbonded term + bonded in one traversal)
+//   -> step2 (Langevin half kick, translational + rotational; cell counts reset).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,37 +46,6 @@ __global__ void k_frames(uint32_t N, const double4* __restrict__ quat, const dou
     fr0[i] = a; fr1[i] = b; fr2[i] = c;
 }
 
-__device__ __forceinline__ uint32_t cell_index(double x, double y, double z, const double* Linv, const int* nc) {
-    double u[3] = {x * Linv[0] + 0.5, y * Linv[1] + 0.5, z * Linv[2] + 0.5};
-    int c[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        double v = u[a] - floor(u[a]);
-        int k = (int)(v * nc[a]);
-        c[a] = k >= nc[a] ? nc[a] - 1 : (k < 0 ? 0 : k);
-    }
-    return ((uint32_t)c[2] * nc[1] + c[1]) * nc[0] + c[0];
-}
-
-struct CellArgs {
-    uint32_t N;
-    const double4* pos;
-    double Linv[3];
-    int nc[3];
-    uint32_t cap;
-    uint32_t* count;
-    uint32_t* members;
-    DevFlags* flags;
-};
-
-__device__ __forceinline__ void bin_particle(const CellArgs& a, uint32_t i, double x, double y, double z) {
-    uint32_t c = cell_index(x, y, z, a.Linv, a.nc);
-    uint32_t slot = atomicAdd(&a.count[c], 1u);
-    if (slot < a.cap) a.members[(size_t)c * a.cap + slot] = i;
-    else atomicOr(&a.flags->overflow, 1u);
-    atomicMax(&a.flags->max_cell, slot + 1);
-}
-
 __global__ void k_cell_assign(CellArgs a) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.N) return;
@@ -89,99 +58,117 @@ __global__ void k_zero_u32(uint32_t* p, size_t n) {
     for (size_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) p[k] = 0u;
 }
 
-struct NlistArgs {
-    uint32_t N;
-    const double4* pos;
-    double L[3], Linv[3];
-    int nc[3];
-    uint32_t cap;
-    const uint32_t* count;
-    const uint32_t* members;
-    const int* bpart;
-    double rlist2;
-    uint32_t max_neigh;
-    uint32_t* n_neigh;
-    uint32_t* nlist;
-    double4* pos_last;
-    DevFlags* flags;
-    int conditional;  // 1: only when flags->rebuild is set
-};
-
-// One thread per particle: scan the (up to) 27 cells around i, keep j != i with
-// |dr|^2 < r_list^2 (FP64, no contraction) that is not bonded to i, then sort the row.
-__global__ void k_nlist_build(NlistArgs a) {
-    if (a.conditional && a.flags->rebuild == 0u) return;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.N) return;
-    if (i == 0) atomicAdd(&a.flags->builds, 1ull);
-    double4 pi = a.pos[i];
-    a.pos_last[i] = pi;
-    int b0 = a.bpart[2 * i], b1 = a.bpart[2 * i + 1];
-    uint32_t c = cell_index(pi.x, pi.y, pi.z, a.Linv, a.nc);
-    int cx = c % a.nc[0], cy = (c / a.nc[0]) % a.nc[1], cz = c / (a.nc[0] * a.nc[1]);
-    int lo[3], hi[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        lo[d] = a.nc[d] >= 3 ? -1 : 0;
-        hi[d] = a.nc[d] >= 3 ? 1 : (a.nc[d] == 2 ? 1 : 0);
-    }
-    uint32_t* row = a.nlist + (size_t)i * a.max_neigh;
-    uint32_t n = 0;
-    for (int dz = lo[2]; dz <= hi[2]; ++dz)
-        for (int dy = lo[1]; dy <= hi[1]; ++dy)
-            for (int dx = lo[0]; dx <= hi[0]; ++dx) {
-                int x = (cx + dx + a.nc[0]) % a.nc[0];
-                int y = (cy + dy + a.nc[1]) % a.nc[1];
-                int z = (cz + dz + a.nc[2]) % a.nc[2];
-                uint32_t cc = ((uint32_t)z * a.nc[1] + y) * a.nc[0] + x;
-                uint32_t m = min(a.count[cc], a.cap);
-                const uint32_t* mem = a.members + (size_t)cc * a.cap;
-                for (uint32_t k = 0; k < m; ++k) {
-                    uint32_t j = mem[k];
-                    if (j == i || (int)j == b0 || (int)j == b1) continue;
-                    double4 pj = a.pos[j];
-                    double ddx = min_image(pi.x - pj.x, a.L[0], a.Linv[0]);
-                    double ddy = min_image(pi.y - pj.y, a.L[1], a.Linv[1]);
-                    double ddz = min_image(pi.z - pj.z, a.L[2], a.Linv[2]);
-                    double r2;
-                    {
+// ------------------------------------------------------------------ neighbour rows
+// Rebuilds particle i's neighbour row inside the force kernel. The TPP lanes of the
+// particle split the (up to) 27 cells around it; each lane counts its matches (FP64
+// minimum-image distance, no contraction, bonded partners excluded), a lane prefix
+// sum places them in the LDS row, and a rank sort writes the row ordered by index to
+// LDS and HBM (deterministic for any order of the cell members). Lanes of one particle
+// are adjacent lanes of one wavefront; every thread of the block reaches the barriers.
+template<int TPP>
+__device__ __forceinline__ bool ref_pair(const NbrArgs& nb, const double4& pi, uint32_t i, uint32_t j, int b0, int b1) {
+    if (j == i || (int)j == b0 || (int)j == b1) return false;
+    const double4 pj = nb.pos[j];
+    double dx = min_image(pi.x - pj.x, nb.L[0], nb.Linv[0]);
+    double dy = min_image(pi.y - pj.y, nb.L[1], nb.Linv[1]);
+    double dz = min_image(pi.z - pj.z, nb.L[2], nb.Linv[2]);
+    double r2;
+    {
 #pragma clang fp contract(off)
-                        r2 = ddx * ddx;
-                        r2 = r2 + ddy * ddy;
-                        r2 = r2 + ddz * ddz;
-                    }
-                    if (r2 < a.rlist2) {
-                        if (n < a.max_neigh) row[n] = j;
-                        ++n;
-                    }
-                }
-            }
-    a.n_neigh[i] = min(n, a.max_neigh);
-    if (n > a.max_neigh) atomicOr(&a.flags->overflow, 2u);
-    atomicMax(&a.flags->max_neigh, n);
-    // insertion sort of the row: deterministic summation order
-    uint32_t m = min(n, a.max_neigh);
-    for (uint32_t p = 1; p < m; ++p) {
-        uint32_t v = row[p];
-        int q = (int)p - 1;
-        while (q >= 0 && row[q] > v) { row[q + 1] = row[q]; --q; }
-        row[q + 1] = v;
+        r2 = dx * dx;
+        r2 = r2 + dy * dy;
+        r2 = r2 + dz * dz;
     }
+    return r2 < nb.rlist2;
+}
+
+template<int TPP>
+__device__ __forceinline__ uint32_t scan_cells(const NbrArgs& nb, const double4& pi, uint32_t i, int b0, int b1,
+                                               uint32_t lane, uint32_t* row, uint32_t base, bool write) {
+    const int nx = nb.nc[0] >= 3 ? 3 : nb.nc[0], ny = nb.nc[1] >= 3 ? 3 : nb.nc[1], nz = nb.nc[2] >= 3 ? 3 : nb.nc[2];
+    const int ox = nb.nc[0] >= 3 ? -1 : 0, oy = nb.nc[1] >= 3 ? -1 : 0, oz = nb.nc[2] >= 3 ? -1 : 0;
+    const uint32_t c = cell_index(pi.x, pi.y, pi.z, nb.Linv, nb.nc);
+    const int cx = c % nb.nc[0], cy = (c / nb.nc[0]) % nb.nc[1], cz = c / (nb.nc[0] * nb.nc[1]);
+    uint32_t n = 0;
+    for (int q = lane; q < nx * ny * nz; q += TPP) {
+        int x = (cx + ox + q % nx + nb.nc[0]) % nb.nc[0];
+        int y = (cy + oy + (q / nx) % ny + nb.nc[1]) % nb.nc[1];
+        int z = (cz + oz + q / (nx * ny) + nb.nc[2]) % nb.nc[2];
+        uint32_t cc = ((uint32_t)z * nb.nc[1] + y) * nb.nc[0] + x;
+        uint32_t m = min(nb.count[cc], nb.cap);
+        const uint32_t* mem = nb.members + (size_t)cc * nb.cap;
+        for (uint32_t k = 0; k < m; ++k) {
+            uint32_t j = mem[k];
+            if (ref_pair<TPP>(nb, pi, i, j, b0, b1)) {
+                if (write && base + n < nb.max_neigh) row[base + n] = j;
+                ++n;
+            }
+        }
+    }
+    return n;
 }
 
 // ------------------------------------------------------------------ fused forces
-template<int TPP, bool WANT_E, bool WANT_V, bool COUNT = false>
+template<int TPP, bool WANT_E, bool WANT_V, bool COUNT>
 __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
+    extern __shared__ uint32_t lds_rows[];
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a.cell_count) {
-        size_t stride = (size_t)gridDim.x * blockDim.x;
-        for (size_t c = tid; c < a.ncell_total; c += stride) a.cell_count[c] = 0u;
-    }
     const uint32_t i = tid / TPP;
     const uint32_t lane = tid % TPP;
     const bool active = i < a.N;
     const DParams& P = *a.P;
     const uint32_t mask = a.mask;
+    const NbrArgs& nb = a.nb;
+    const bool rebuild = nb.force_rebuild || (nb.flags && nb.flags->rebuild);
+
+    uint32_t nn = 0;
+    const uint32_t* row = nullptr;
+    double4 pi = make_double4(0, 0, 0, 0);
+    if (active) pi = a.pos[i];
+    if (rebuild) {
+        // ---- neighbour row of i (uniform branch: every thread of the block is here)
+        const uint32_t slot = threadIdx.x / TPP;
+        uint32_t* urow = lds_rows + (size_t)slot * 2 * nb.max_neigh;
+        uint32_t* srow = urow + nb.max_neigh;
+        int b0 = -1, b1 = -1;
+        uint32_t cnt = 0;
+        if (active) {
+            b0 = a.bpart[2 * i]; b1 = a.bpart[2 * i + 1];
+            cnt = scan_cells<TPP>(nb, pi, i, b0, b1, lane, urow, 0, false);
+        }
+        uint32_t v = cnt;
+#pragma unroll
+        for (int d = 1; d < TPP; d <<= 1) {
+            uint32_t y = __shfl_up(v, d, TPP);
+            if ((int)lane >= d) v += y;
+        }
+        const uint32_t total = __shfl(v, TPP - 1, TPP);
+        if (active) scan_cells<TPP>(nb, pi, i, b0, b1, lane, urow, v - cnt, true);
+        __syncthreads();
+        const uint32_t m = min(total, nb.max_neigh);
+        if (active) {
+            for (uint32_t e = lane; e < m; e += TPP) {
+                uint32_t x = urow[e], r = 0;
+                for (uint32_t q = 0; q < m; ++q) r += urow[q] < x ? 1u : 0u;
+                srow[r] = x;
+                a.nlist[a.head[i] + r] = x;
+            }
+            if (lane == 0) {
+                a.n_neigh[i] = m;
+                nb.pos_last[i] = pi;
+                if (total > nb.max_neigh) atomicOr(&nb.flags->overflow, 2u);
+                atomicMax(&nb.flags->max_neigh, total);
+                if (i == 0) atomicAdd(&nb.flags->builds, 1ull);
+            }
+        }
+        __syncthreads();
+        nn = m;
+        row = srow;
+    } else if (active) {
+        nn = a.n_neigh[i];
+        row = a.nlist + a.head[i];
+    }
+
     V F = zero3(), T = zero3();
     float Eh = 0.f;
     float et[8];
@@ -194,16 +181,13 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
 #pragma unroll
     for (int s = 0; s < 14; ++s) cnt[s] = 0u;
     if (active) {
-        const double4 pi = a.pos[i];
         const float4 f0 = a.fr0[i], f1 = a.fr1[i], f2 = a.fr2[i];
         Frame fi{mk(f0.x, f0.y, f0.z), mk(f1.x, f1.y, f1.z), mk(f2.x, f2.y, f2.z)};
         const float qi = f0.w;
         const int ti = __float_as_int(f1.w);
         const int nt = P.ntypes;
-        const uint32_t nn = a.n_neigh[i];
-        const size_t h = a.head[i];
         for (uint32_t k = lane; k < nn; k += TPP) {
-            const uint32_t j = a.nlist[h + k];
+            const uint32_t j = row[k];
             const double4 pj = a.pos[j];
             const float4 g0 = a.fr0[j], g1 = a.fr1[j], g2 = a.fr2[j];
             Frame fj{mk(g0.x, g0.y, g0.z), mk(g1.x, g1.y, g1.z), mk(g2.x, g2.y, g2.z)};
@@ -278,7 +262,8 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
             }
         }
         if (mask & (1u << T_BONDED)) {
-            for (uint32_t b = lane; b < 2; b += TPP) {
+            // bond slots go to the last lanes, which hold the fewest pairs
+            for (uint32_t b = (TPP - 1) - lane; b < 2; b += TPP) {
                 const int other = a.bpart[2 * i + b];
                 if (other < 0) continue;
                 const int info = a.binfo[2 * i + b];
@@ -317,7 +302,7 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
         for (int s = 0; s < 14; ++s)
             if (cnt[s]) atomicAdd(&a.counts[s], (unsigned long long)cnt[s]);
     }
-    // reduce the TPP lanes of a particle (they are adjacent lanes of one wavefront)
+    // reduce the TPP lanes of a particle (adjacent lanes of one wavefront)
 #pragma unroll
     for (int m = TPP / 2; m > 0; m >>= 1) {
         F.x += __shfl_xor(F.x, m); F.y += __shfl_xor(F.y, m); F.z += __shfl_xor(F.z, m);
@@ -328,7 +313,7 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
             for (int s = 0; s < 6; ++s) vir[s] += __shfl_xor(vir[s], m);
         }
     }
-    if (a.split) {
+    if (WANT_E && a.split) {
         float s0 = et[S_FENE], s1 = et[S_BEXC], s2 = et[S_STACK];
 #pragma unroll
         for (int m = TPP / 2; m > 0; m >>= 1) {
@@ -404,7 +389,7 @@ __device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) 
         k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
     }
 }
-__device__ __forceinline__ double u01(uint32_t x) { return ((double)x + 0.5) * (1.0 / 4294967296.0); }
+__device__ __forceinline__ float u01f(uint32_t x) { return ((float)(x >> 8) + 0.5f) * (1.0f / 16777216.0f); }
 
 // q1 (x) q2 for quaternions stored (w, x, y, z) in double4 (x=w, y=x, z=y, w=z).
 __device__ __forceinline__ double4 qmul(double4 a, double4 b) {
@@ -423,6 +408,19 @@ __device__ __forceinline__ void to_lab(double4 q, double vx, double vy, double v
     double4 t = qmul(qmul(q, make_double4(0.0, vx, vy, vz)), qconj(q));
     bx = t.y; by = t.z; bz = t.w;
 }
+// cos and sin of x: Taylor series through x^13 for |x| <= 1/4 (truncation below
+// 2^-60, i.e. exact in FP64), the library functions otherwise.
+__device__ __forceinline__ void sincos_small(double x, double& s, double& c) {
+    if (fabs(x) <= 0.25) {
+        double x2 = x * x;
+        s = x * (1.0 + x2 * (-1.0 / 6 + x2 * (1.0 / 120 + x2 * (-1.0 / 5040 + x2 * (1.0 / 362880 +
+                 x2 * (-1.0 / 39916800 + x2 * (1.0 / 6227020800.0)))))));
+        c = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24 + x2 * (-1.0 / 720 + x2 * (1.0 / 40320 + x2 * (-1.0 / 3628800 +
+                 x2 * (1.0 / 479001600.0))))));
+    } else {
+        sincos(x, &s, &c);
+    }
+}
 // Free-rotor sub-step about body axis k for time tau (Miller et al., J. Chem. Phys.
 // 116, 8649 (2002), NO_SQUISH): P_k permutes quaternion components.
 __device__ __forceinline__ double4 permute(int k, double4 q) {
@@ -433,40 +431,18 @@ __device__ __forceinline__ double4 permute(int k, double4 q) {
 __device__ __forceinline__ void rotor(int k, double tau, double Ik, double4& q, double4& p) {
     double4 Pq = permute(k, q), Pp = permute(k, p);
     double zeta = (p.x * Pq.x + p.y * Pq.y + p.z * Pq.z + p.w * Pq.w) / (4.0 * Ik);
-    double c = cos(zeta * tau), s = sin(zeta * tau);
+    double s, c;
+    sincos_small(zeta * tau, s, c);
     q = make_double4(c * q.x + s * Pq.x, c * q.y + s * Pq.y, c * q.z + s * Pq.z, c * q.w + s * Pq.w);
     p = make_double4(c * p.x + s * Pp.x, c * p.y + s * Pp.y, c * p.z + s * Pp.z, c * p.w + s * Pp.w);
 }
 
-struct StepArgs {
-    uint32_t N;
-    double4* bd;       // Langevin force (xyz) of the last second half-kick, lab frame
-    double4* bd_tq;    // Langevin torque (xyz) of the last second half-kick, lab frame
-    double4* pos;
-    double4* vel;
-    double4* quat;
-    double4* angmom;
-    const float4* force;
-    const float4* torque;
-    const double* charge;
-    float4* fr0;
-    float4* fr1;
-    float4* fr2;
-    const double4* pos_last;
-    double dt, kT, gamma, gamma_r, mass, inertia;
-    double L[3], Linv[3];
-    double skin_half2;
-    unsigned long long seed;
-    DevFlags* flags;
-    CellArgs cells;
-};
-
 // First half of the step: v += dt/2 (F + F_L)/m, x += dt v (wrapped),
 // p += dt q (x) (tau + tau_L)_body, where F_L / tau_L are the Langevin force and torque
 // drawn in the previous second half (velocity Verlet on the total force, as the
-// host engine's Langevin method does),
-// NO_SQUISH rotation (3: dt/2, 2: dt/2, 1: dt, 2: dt/2, 3: dt/2), renormalised q; then
-// the new frames, cell binning, and the skin test that requests a rebuild.
+// host engine's Langevin method does), NO_SQUISH rotation (3: dt/2, 2: dt/2, 1: dt,
+// 2: dt/2, 3: dt/2), renormalised q; then the new frames, cell binning, and the skin
+// test that requests a neighbour-list rebuild.
 __global__ void k_step1(StepArgs a) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) a.flags->step += 1ull;
@@ -502,17 +478,20 @@ __global__ void k_step1(StepArgs a) {
     f1.w = __int_as_float(type_of(x.w));
     f2.w = 0.f;
     a.fr0[i] = f0; a.fr1[i] = f1; a.fr2[i] = f2;
-    bin_particle(a.cells, i, x.x, x.y, x.z);
-    double4 l = a.pos_last[i];
-    double dx = min_image(x.x - l.x, a.L[0], a.Linv[0]);
-    double dy = min_image(x.y - l.y, a.L[1], a.Linv[1]);
-    double dz = min_image(x.z - l.z, a.L[2], a.Linv[2]);
-    if (dx * dx + dy * dy + dz * dz > a.skin_half2) atomicOr(&a.flags->rebuild, 1u);
+    if (a.cells.count) {
+        bin_particle(a.cells, i, x.x, x.y, x.z);
+        double4 l = a.pos_last[i];
+        double dx = min_image(x.x - l.x, a.L[0], a.Linv[0]);
+        double dy = min_image(x.y - l.y, a.L[1], a.Linv[1]);
+        double dz = min_image(x.z - l.z, a.L[2], a.Linv[2]);
+        if (dx * dx + dy * dy + dz * dz > a.skin_half2) atomicOr(&a.flags->rebuild, 1u);
+    }
 }
 
 // Second half: Langevin forces (drag -gamma v plus uniform noise of variance
 // 2 gamma kT / dt per component; rotational drag -gamma_r L/I plus Gaussian noise of
 // variance 2 gamma_r kT / dt in the body frame), then v += dt/2 F/m, p += dt q (x) tau.
+// Also clears this particle's cell count for the next step's binning.
 __global__ void k_step2(StepArgs a) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) a.flags->rebuild = 0u;
@@ -524,11 +503,13 @@ __global__ void k_step2(StepArgs a) {
     uint32_t d[4] = {i, (uint32_t)step, (uint32_t)(step >> 32), 1u};
     philox(d, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
     double4 v = a.vel[i];
+    const double4 x = a.pos[i];
+    if (a.cells.count) a.cells.count[cell_index(x.x, x.y, x.z, a.cells.Linv, a.cells.nc)] = 0u;
     float4 f = a.force[i];
     const double ct = sqrt(6.0 * a.gamma * a.kT / dt);
-    double bx = -a.gamma * v.x + ct * (2.0 * u01(c[0]) - 1.0);
-    double by = -a.gamma * v.y + ct * (2.0 * u01(c[1]) - 1.0);
-    double bz = -a.gamma * v.z + ct * (2.0 * u01(c[2]) - 1.0);
+    double bx = -a.gamma * v.x + ct * (2.0 * u01f(c[0]) - 1.0);
+    double by = -a.gamma * v.y + ct * (2.0 * u01f(c[1]) - 1.0);
+    double bz = -a.gamma * v.z + ct * (2.0 * u01f(c[2]) - 1.0);
     const double hm = 0.5 * dt / a.mass;
     v.x += hm * (f.x + bx); v.y += hm * (f.y + by); v.z += hm * (f.z + bz);
     a.vel[i] = v;
@@ -540,13 +521,14 @@ __global__ void k_step2(StepArgs a) {
     double4 Lb = qmul(qconj(q), p);  // = 2 (0, L_body)
     const double I = a.inertia;
     const double cr = sqrt(2.0 * a.gamma_r * a.kT / dt);
-    double r1 = sqrt(-2.0 * log(u01(c[3]))), r2 = sqrt(-2.0 * log(u01(d[1])));
-    double g0 = r1 * cos(6.283185307179586 * u01(d[0]));
-    double g1 = r1 * sin(6.283185307179586 * u01(d[0]));
-    double g2 = r2 * cos(6.283185307179586 * u01(d[2]));
-    double rbx = cr * g0 - a.gamma_r * (0.5 * Lb.y) / I;
-    double rby = cr * g1 - a.gamma_r * (0.5 * Lb.z) / I;
-    double rbz = cr * g2 - a.gamma_r * (0.5 * Lb.w) / I;
+    // Box-Muller in FP32: the noise needs no FP64 accuracy
+    float r1 = sqrtf(-2.f * __logf(u01f(c[3]))), r2 = sqrtf(-2.f * __logf(u01f(d[1])));
+    float s0, c0, s2, c2;
+    __sincosf(6.2831853f * u01f(d[0]), &s0, &c0);
+    __sincosf(6.2831853f * u01f(d[2]), &s2, &c2);
+    double rbx = cr * (double)(r1 * c0) - a.gamma_r * (0.5 * Lb.y) / I;
+    double rby = cr * (double)(r1 * s0) - a.gamma_r * (0.5 * Lb.z) / I;
+    double rbz = cr * (double)(r2 * c2) - a.gamma_r * (0.5 * Lb.w) / I;
     tbx += rbx; tby += rby; tbz += rbz;
     double lx, ly, lz;
     to_lab(q, rbx, rby, rbz, lx, ly, lz);
@@ -593,7 +575,9 @@ template<class T> struct DBuf {
 };
 
 constexpr int kBlock = 256;
-constexpr int kTPP = 8;
+constexpr int kTPPMax = 32;
+constexpr size_t kMaxLds = 48 * 1024;
+constexpr uint32_t kMaxNeigh = 768;  // 8 particles x 2 rows x 768 x 4 B = 48 KiB of LDS at 32 lanes
 
 }  // namespace
 
@@ -636,6 +620,9 @@ struct oxdna_ctx {
     double dt = 0.003, kT = 0.1, gamma = 1.0, gamma_r = 1.0;
     uint64_t seed = 0;
 
+    int tpp = 16;                        // force-kernel lanes per particle (autotuned)
+    uint32_t tuned_mask = 0xffffffffu;
+    uint32_t tuned_n = 0;
     uint32_t forces_mask = 0xffffffffu;  // mask of the forces currently in `force`
     hipGraphExec_t graph = nullptr;
     uint32_t graph_mask = 0;
@@ -844,7 +831,40 @@ void upload_params(oxdna_ctx* c) {
     c->params_dirty = false;
 }
 
+// ------------------------------------------------------------------ force launches
+// One templated launcher for every instantiation: TPP lanes per particle (8, 16, 32).
+void check_force_args(const ForceArgs& a);
+template<bool E, bool V, bool CNT>
+void launch_k(int tpp, dim3 g, dim3 b, size_t lds, hipStream_t st, const ForceArgs& a) {
+    check_force_args(a);
+    switch (tpp) {
+    case 8: hipLaunchKernelGGL((k_forces<8, E, V, CNT>), g, b, lds, st, a); break;
+    case 16: hipLaunchKernelGGL((k_forces<16, E, V, CNT>), g, b, lds, st, a); break;
+    default: hipLaunchKernelGGL((k_forces<32, E, V, CNT>), g, b, lds, st, a); break;
+    }
+}
+void check_force_args(const ForceArgs& a) {
+    if (a.N == 0) return;
+    if (!a.pos || !a.fr0 || !a.fr1 || !a.fr2 || !a.P || !a.force || !a.torque || !a.n_neigh || !a.nlist || !a.head)
+        throw Fail("internal: force kernel launched with an unset array");
+    if ((a.mask & (1u << T_BONDED)) && (!a.bpart || !a.binfo)) throw Fail("internal: bonded launch without bond table");
+    if (a.nb.flags && (!a.nb.count || !a.nb.members || !a.nb.pos_last || !a.bpart))
+        throw Fail("internal: row rebuild without a cell list");
+}
+void launch_k_any(int tpp, bool e, bool v, dim3 g, dim3 b, size_t lds, hipStream_t st, const ForceArgs& a) {
+    check_force_args(a);
+    if (e && v) launch_k<true, true, false>(tpp, g, b, lds, st, a);
+    else if (e) launch_k<true, false, false>(tpp, g, b, lds, st, a);
+    else if (v) launch_k<false, true, false>(tpp, g, b, lds, st, a);
+    else launch_k<false, false, false>(tpp, g, b, lds, st, a);
+}
+
 // ------------------------------------------------------------------ neighbour list management
+constexpr size_t kMaxCells = size_t(1) << 24;
+
+// LDS of the in-kernel row rebuild: two rows (unsorted, sorted) per particle of a block.
+size_t lds_bytes(const oxdna_ctx* c, int tpp) { return (size_t)(kBlock / tpp) * 2 * c->max_neigh * sizeof(uint32_t); }
+
 void setup_cells(oxdna_ctx* c) {
     // neighbour radius: the largest cut-off of every attached nonbonded term + skin
     double rmax = pair_rmax(c, 0x3f);
@@ -853,8 +873,10 @@ void setup_cells(oxdna_ctx* c) {
     for (int a = 0; a < 3; ++a) {
         int n = c->rlist > 0 ? (int)std::floor(c->L[a] / c->rlist) : 1;
         c->nc[a] = std::max(1, std::min(n, 1024));
-        total *= (size_t)c->nc[a];
     }
+    // bound the cell count (cells may only grow, never drop below r_list)
+    while ((total = (size_t)c->nc[0] * c->nc[1] * c->nc[2]) > kMaxCells)
+        for (int a = 0; a < 3; ++a) c->nc[a] = std::max(1, c->nc[a] / 2);
     if (c->cell_cap == 0) {
         double per_cell = (double)c->N / (double)total;
         c->cell_cap = (uint32_t)std::max(16.0, std::ceil(per_cell * 4.0 + 8.0));
@@ -864,43 +886,27 @@ void setup_cells(oxdna_ctx* c) {
     hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for(total), 4096)), dim3(kBlock), 0, c->stream,
                        c->cell_count.p, total);
     if (c->max_neigh == 0) c->max_neigh = 64;
+    if (c->max_neigh > kMaxNeigh) throw Fail("neighbour rows would exceed " + std::to_string(kMaxNeigh) + " entries");
     c->nlist.alloc((size_t)c->N * c->max_neigh);
     std::vector<size_t> h(c->N);
     for (uint32_t i = 0; i < c->N; ++i) h[i] = (size_t)i * c->max_neigh;
     c->head.alloc(c->N);
     HIP_OK(hipMemcpyAsync(c->head.p, h.data(), sizeof(size_t) * c->N, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    if (lds_bytes(c, c->tpp) > kMaxLds) c->tpp = 32;
 }
 
 size_t ncell_total(const oxdna_ctx* c) { return (size_t)c->nc[0] * c->nc[1] * c->nc[2]; }
 
-CellArgs cell_args(oxdna_ctx* c) {
+CellArgs cell_args(oxdna_ctx* c, bool with_cells) {
     CellArgs a;
     a.N = c->N;
     a.pos = c->pos.p;
     for (int k = 0; k < 3; ++k) { a.Linv[k] = c->Linv[k]; a.nc[k] = c->nc[k]; }
     a.cap = c->cell_cap;
-    a.count = c->cell_count.p;
+    a.count = with_cells ? c->cell_count.p : nullptr;
     a.members = c->cell_members.p;
     a.flags = c->flags.p;
-    return a;
-}
-
-NlistArgs nlist_args(oxdna_ctx* c, int conditional) {
-    NlistArgs a;
-    a.N = c->N;
-    a.pos = c->pos.p;
-    for (int k = 0; k < 3; ++k) { a.L[k] = c->L[k]; a.Linv[k] = c->Linv[k]; a.nc[k] = c->nc[k]; }
-    a.cap = c->cell_cap;
-    a.count = c->cell_count.p;
-    a.members = c->cell_members.p;
-    a.bpart = c->bpart.p;
-    a.rlist2 = c->rlist * c->rlist;
-    a.max_neigh = c->max_neigh;
-    a.n_neigh = c->n_neigh.p;
-    a.nlist = c->nlist.p;
-    a.pos_last = c->pos_last.p;
-    a.flags = c->flags.p;
-    a.conditional = conditional;
     return a;
 }
 
@@ -916,45 +922,6 @@ void clear_overflow(oxdna_ctx* c) {
     HIP_OK(hipMemcpyAsync(c->flags.p, &f, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
 }
 
-// Unconditional rebuild from the current positions, growing buffers until nothing overflows.
-void build_nlist(oxdna_ctx* c) {
-    if (c->rlist <= 0.0) setup_cells(c);
-    for (int attempt = 0; attempt < 8; ++attempt) {
-        clear_overflow(c);
-        hipLaunchKernelGGL(k_cell_assign, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, cell_args(c));
-        hipLaunchKernelGGL(k_nlist_build, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, nlist_args(c, 0));
-        size_t nct = ncell_total(c);
-        hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for(nct), 4096)), dim3(kBlock), 0, c->stream,
-                           c->cell_count.p, nct);
-        DevFlags f;
-        read_flags(c, f);
-        if (f.overflow == 0) {
-            // keep headroom so that rows can grow between rebuilds during a run
-            uint32_t want = std::max<uint32_t>(16, (uint32_t)std::ceil(f.max_neigh * 1.5) + 8);
-            if (want > c->max_neigh) {
-                c->max_neigh = want;
-                c->drop_graph();
-                setup_cells(c);
-                continue;
-            }
-            uint32_t want_cap = (uint32_t)std::ceil(f.max_cell * 1.5) + 4;
-            if (want_cap > c->cell_cap) {
-                c->cell_cap = want_cap;
-                c->drop_graph();
-                setup_cells(c);
-                continue;
-            }
-            c->nlist_valid = true;
-            return;
-        }
-        if (f.overflow & 1u) c->cell_cap = (uint32_t)std::ceil(f.max_cell * 1.5) + 4;
-        if (f.overflow & 2u) c->max_neigh = (uint32_t)std::ceil(f.max_neigh * 1.5) + 8;
-        c->drop_graph();
-        setup_cells(c);
-    }
-    throw Fail("neighbour list could not be sized");
-}
-
 void ensure_ready(oxdna_ctx* c) {
     if (c->N == 0) throw Fail("no particles: call oxdna_set_state first");
     if (c->params_dirty) {
@@ -965,7 +932,7 @@ void ensure_ready(oxdna_ctx* c) {
     upload_params(c);
 }
 
-ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, bool zero_cells) {
+ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild) {
     ForceArgs a;
     a.N = c->N;
     a.pos = c->pos.p;
@@ -984,25 +951,84 @@ ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, bool zero_cell
     a.vpitch = c->N;
     a.epart = c->epart.p;
     a.accumulate = accumulate;
-    a.cell_count = zero_cells ? c->cell_count.p : nullptr;
-    a.ncell_total = (uint32_t)ncell_total(c);
     a.split = nullptr;
     a.counts = nullptr;
+    NbrArgs& nb = a.nb;
+    nb.pos = c->pos.p;
+    for (int k = 0; k < 3; ++k) { nb.L[k] = c->L[k]; nb.Linv[k] = c->Linv[k]; nb.nc[k] = c->nc[k]; }
+    nb.cap = c->cell_cap;
+    nb.count = c->cell_count.p;
+    nb.members = c->cell_members.p;
+    nb.rlist2 = c->rlist * c->rlist;
+    nb.max_neigh = c->max_neigh;
+    nb.pos_last = c->pos_last.p;
+    // rebuild: 0 never (rows are current), 1 always, 2 when the skin test asked for it
+    nb.flags = (rebuild != 0 && (mask & 0x3fu)) ? c->flags.p : nullptr;
+    nb.force_rebuild = (rebuild == 1 && (mask & 0x3fu)) ? 1 : 0;
     return a;
 }
 
-uint32_t force_blocks(const oxdna_ctx* c) { return blocks_for((size_t)c->N * kTPP); }
+uint32_t force_blocks(const oxdna_ctx* c, int tpp) { return blocks_for((size_t)c->N * tpp); }
 
-void launch_forces(oxdna_ctx* c, uint32_t mask, int accumulate, bool want_e, bool want_v, bool zero_cells) {
-    ForceArgs a = force_args(c, mask, accumulate, zero_cells);
-    dim3 grid(force_blocks(c)), block(kBlock);
-    if (want_e && want_v) hipLaunchKernelGGL((k_forces<kTPP, true, true>), grid, block, 0, c->stream, a);
-    else if (want_e) hipLaunchKernelGGL((k_forces<kTPP, true, false>), grid, block, 0, c->stream, a);
-    else if (want_v) hipLaunchKernelGGL((k_forces<kTPP, false, true>), grid, block, 0, c->stream, a);
-    else hipLaunchKernelGGL((k_forces<kTPP, false, false>), grid, block, 0, c->stream, a);
+void launch_forces(oxdna_ctx* c, uint32_t mask, int accumulate, bool want_e, bool want_v, int rebuild) {
+    ForceArgs a = force_args(c, mask, accumulate, rebuild);
+    dim3 grid(force_blocks(c, c->tpp)), block(kBlock);
+    size_t lds = a.nb.flags ? lds_bytes(c, c->tpp) : 0;
+    launch_k_any(c->tpp, want_e, want_v, grid, block, lds, c->stream, a);
     if (want_e)
         hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(256), 0, c->stream, grid.x, c->epart.p, c->energy.p,
                            accumulate);
+}
+
+// Evaluates `m` at the current state, rebuilding every neighbour row first; grows
+// the cell / row capacities (with headroom for the rows to grow during a run) until
+// nothing overflows.
+void compute_rebuild(oxdna_ctx* c, uint32_t m, int fused) {
+    bool pairs = (m & 0x3fu) != 0;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        if (pairs) {
+            clear_overflow(c);
+            hipLaunchKernelGGL(k_cell_assign, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, cell_args(c, true));
+        }
+        if (fused || m == 0) {
+            launch_forces(c, m, 0, true, true, pairs ? 1 : 0);
+        } else {
+            int first = 1;
+            for (int t = 0; t < OXDNA_N_TERMS; ++t) {
+                if (!((m >> t) & 1u)) continue;
+                launch_forces(c, 1u << t, first ? 0 : 1, true, true, (first && pairs) ? 1 : 0);
+                first = 0;
+            }
+            if (first) launch_forces(c, 0, 0, true, true, 0);
+        }
+        if (!pairs) {
+            c->nlist_valid = true;
+            return;
+        }
+        size_t nct = ncell_total(c);
+        hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for(nct), 4096)), dim3(kBlock), 0, c->stream,
+                           c->cell_count.p, nct);
+        DevFlags f;
+        read_flags(c, f);
+        bool grow = false;
+        uint32_t want_cap = (uint32_t)std::ceil(f.max_cell * 1.5) + 4;
+        if (want_cap > c->cell_cap) {
+            c->cell_cap = want_cap;
+            grow = true;
+        }
+        uint32_t want = std::max<uint32_t>(16, (uint32_t)std::ceil(f.max_neigh * 1.5) + 8);
+        if (want > c->max_neigh) {
+            c->max_neigh = want;
+            grow = true;
+        }
+        if (!grow && f.overflow == 0) {
+            c->nlist_valid = true;
+            return;
+        }
+        c->drop_graph();
+        setup_cells(c);
+    }
+    throw Fail("neighbour list could not be sized");
 }
 
 void compute(oxdna_ctx* c, uint32_t mask, int fused) {
@@ -1010,21 +1036,11 @@ void compute(oxdna_ctx* c, uint32_t mask, int fused) {
     hipLaunchKernelGGL(k_frames, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, c->N, c->quat.p, c->charge.p,
                        c->pos.p, c->fr0.p, c->fr1.p, c->fr2.p);
     uint32_t m = effective_mask(c, mask);
-    if (m & 0x3fu) build_nlist(c);
-    if (fused || m == 0) {
-        launch_forces(c, m, 0, true, true, false);
-    } else {
-        int first = 1;
-        for (int t = 0; t < OXDNA_N_TERMS; ++t) {
-            if (!((m >> t) & 1u)) continue;
-            launch_forces(c, 1u << t, first ? 0 : 1, true, true, false);
-            first = 0;
-        }
-    }
+    compute_rebuild(c, m, fused);
     c->forces_mask = m;
 }
 
-StepArgs step_args(oxdna_ctx* c) {
+StepArgs step_args(oxdna_ctx* c, bool with_cells) {
     StepArgs a;
     a.N = c->N;
     a.bd = c->bd.p; a.bd_tq = c->bd_tq.p;
@@ -1037,16 +1053,17 @@ StepArgs step_args(oxdna_ctx* c) {
     a.skin_half2 = 0.25 * c->buffer * c->buffer;
     a.seed = c->seed;
     a.flags = c->flags.p;
-    a.cells = cell_args(c);
+    a.cells = cell_args(c, with_cells);
     return a;
 }
 
+// One step: step1 (kick, drift, rotate, frames, binning, skin test) -> forces (rows
+// rebuilt in-kernel when the skin test fired) -> step2 (Langevin kick, cell reset).
 void enqueue_step(oxdna_ctx* c, uint32_t mask) {
-    StepArgs s = step_args(c);
+    bool pairs = (mask & 0x3fu) != 0;
+    StepArgs s = step_args(c, pairs);
     hipLaunchKernelGGL(k_step1, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, s);
-    if (mask & 0x3fu)
-        hipLaunchKernelGGL(k_nlist_build, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, nlist_args(c, 1));
-    launch_forces(c, mask, 0, false, false, true);
+    launch_forces(c, mask, 0, false, false, pairs ? 2 : 0);
     hipLaunchKernelGGL(k_step2, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, s);
 }
 
@@ -1062,6 +1079,37 @@ void build_graph(oxdna_ctx* c, uint32_t mask) {
     HIP_OK(hipGraphDestroy(g));
     c->graph_mask = mask;
     c->graph_steps = kGraphSteps;
+}
+
+// Autotuner: lanes per particle (8, 16, 32) of the force kernel, timed at the
+// current state (rows current, no rebuild) with HIP events; keeps the fastest.
+void autotune(oxdna_ctx* c, uint32_t m) {
+    if (c->tuned_mask == m && c->tuned_n == c->N) return;
+    const int cand[3] = {8, 16, 32};
+    float best = 1e30f;
+    int best_tpp = c->tpp;
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    for (int tpp : cand) {
+        if (lds_bytes(c, tpp) > kMaxLds) continue;
+        c->tpp = tpp;
+        for (int w = 0; w < 3; ++w) launch_forces(c, m, 0, false, false, 0);
+        HIP_OK(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < 20; ++r) launch_forces(c, m, 0, false, false, 0);
+        HIP_OK(hipEventRecord(e1, c->stream));
+        HIP_OK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; best_tpp = tpp; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->tpp = best_tpp;
+    c->tuned_mask = m;
+    c->tuned_n = c->N;
+    c->drop_graph();
+    // the timing launches overwrote the forces with identical values; nothing to redo
 }
 
 struct Saved {
@@ -1095,6 +1143,7 @@ void run(oxdna_ctx* c, uint32_t nsteps, uint32_t mask, int use_graph) {
     ensure_ready(c);
     uint32_t m = effective_mask(c, mask);
     if (c->forces_mask != m || !c->nlist_valid) compute(c, m, 1);
+    autotune(c, m);
     const uint32_t chunk_max = 4096;
     uint32_t done = 0;
     Saved saved;
@@ -1120,14 +1169,13 @@ void run(oxdna_ctx* c, uint32_t nsteps, uint32_t mask, int use_graph) {
             if (attempt >= 4) throw Fail("neighbour list kept overflowing during the run");
             // grow, restore the chunk's starting state and run it again
             restore_state(c, saved);
-            f.overflow = 0; f.step = step0; f.rebuild = 0;
-            HIP_OK(hipMemcpyAsync(c->flags.p, &f, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
             c->max_neigh = std::max(c->max_neigh, (uint32_t)std::ceil(f.max_neigh * 1.5) + 8);
             c->cell_cap = std::max(c->cell_cap, (uint32_t)std::ceil(f.max_cell * 1.5) + 4);
+            f.overflow = 0; f.step = step0; f.rebuild = 0;
+            HIP_OK(hipMemcpyAsync(c->flags.p, &f, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
             c->drop_graph();
             setup_cells(c);
-            HIP_OK(hipStreamSynchronize(c->stream));
-            build_nlist(c);
+            compute(c, m, 1);
         }
         done += chunk;
     }
@@ -1224,7 +1272,7 @@ int oxdna_set_state(oxdna_ctx* c, unsigned int n, const double* pos, const doubl
         c->bd.alloc(n); c->bd_tq.alloc(n);
         c->charge.alloc(n); c->fr0.alloc(n); c->fr1.alloc(n); c->fr2.alloc(n);
         c->force.alloc(n); c->torque.alloc(n); c->virial.alloc(6 * (size_t)n); c->n_neigh.alloc(n);
-        c->epart.alloc((size_t)blocks_for((size_t)n * kTPP) * 8);
+        c->epart.alloc((size_t)blocks_for((size_t)n * kTPPMax) * 8);
         HIP_OK(hipMemcpyAsync(c->pos.p, p.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
         HIP_OK(hipMemcpyAsync(c->pos_last.p, p.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
         HIP_OK(hipMemcpyAsync(c->quat.p, q.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
@@ -1393,10 +1441,7 @@ int oxdna_get_nlist_pairs(oxdna_ctx* c, int32_t* pairs, size_t max_pairs, size_t
     if (!c) return fail(nullptr, "NULL context");
     int rc = 0;
     int g = guard(c, [&] {
-        if (!c->nlist_valid) {
-            ensure_ready(c);
-            build_nlist(c);
-        }
+        if (!c->nlist_valid) compute(c, 0x3fu, 1);
         uint32_t n = c->N;
         std::vector<uint32_t> nn(n), rows((size_t)n * c->max_neigh);
         HIP_OK(hipMemcpyAsync(nn.data(), c->n_neigh.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
@@ -1492,9 +1537,9 @@ int oxdna_count_evaluations(oxdna_ctx* c, unsigned int mask, uint64_t out[14]) {
         DBuf<unsigned long long> d;
         d.alloc(14);
         HIP_OK(hipMemsetAsync(d.p, 0, 14 * sizeof(unsigned long long), c->stream));
-        ForceArgs a = force_args(c, m, 0, false);
+        ForceArgs a = force_args(c, m, 0, 0);
         a.counts = d.p;
-        hipLaunchKernelGGL((k_forces<kTPP, false, false, true>), dim3(force_blocks(c)), dim3(kBlock), 0, c->stream, a);
+        launch_k<false, false, true>(c->tpp, dim3(force_blocks(c, c->tpp)), dim3(kBlock), 0, c->stream, a);
         unsigned long long h[14];
         HIP_OK(hipMemcpyAsync(h, d.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
         HIP_OK(hipStreamSynchronize(c->stream));
@@ -1603,13 +1648,17 @@ int oxdna_pair_forces(int term, float* d_force, float* d_torque, float* d_virial
         P.mask = 1u << term;
         const DParams* dP = stage_params(s, P, st);
         driver_frames(s, N, d_pos, d_orientation, d_charge, st);
-        uint32_t blocks = blocks_for((size_t)N * kTPP);
+        const int tpp = 16;
+        uint32_t blocks = blocks_for((size_t)N * tpp);
         s.epart.alloc((size_t)blocks * 8);
         ForceArgs a{};
         a.N = N;
         a.pos = (const double4*)d_pos;
         a.fr0 = s.fr0.p; a.fr1 = s.fr1.p; a.fr2 = s.fr2.p;
-        a.n_neigh = d_n_neigh; a.nlist = d_nlist; a.head = d_head_list;
+        // the caller's rows are only read (no rebuild without a flags block)
+        a.n_neigh = const_cast<uint32_t*>(d_n_neigh);
+        a.nlist = const_cast<uint32_t*>(d_nlist);
+        a.head = d_head_list;
         a.P = dP;
         for (int k = 0; k < 3; ++k) { a.L[k] = box->L[k]; a.Linv[k] = 1.0 / box->L[k]; }
         a.mask = 1u << term;
@@ -1617,11 +1666,7 @@ int oxdna_pair_forces(int term, float* d_force, float* d_torque, float* d_virial
         a.virial = d_virial; a.vpitch = virial_pitch;
         a.epart = s.epart.p;
         bool we = d_term_energy != nullptr, wv = d_virial != nullptr;
-        dim3 g(blocks), b(kBlock);
-        if (we && wv) hipLaunchKernelGGL((k_forces<kTPP, true, true>), g, b, 0, st, a);
-        else if (we) hipLaunchKernelGGL((k_forces<kTPP, true, false>), g, b, 0, st, a);
-        else if (wv) hipLaunchKernelGGL((k_forces<kTPP, false, true>), g, b, 0, st, a);
-        else hipLaunchKernelGGL((k_forces<kTPP, false, false>), g, b, 0, st, a);
+        launch_k_any(tpp, we, wv, dim3(blocks), dim3(kBlock), 0, st, a);
         if (we) hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(256), 0, st, blocks, s.epart.p, d_term_energy, 0);
         HIP_OK(hipGetLastError());
     });
@@ -1673,11 +1718,12 @@ int oxdna_bonded_forces(float* d_force, float* d_torque, float* d_virial, size_t
         a.n_neigh = s.zeros.p;
         a.nlist = s.zeros.p;
         a.head = s.zhead.p;
-        s.epart.alloc((size_t)blocks_for((size_t)N * kTPP) * 8);
+        const int tpp = 16;
+        s.epart.alloc((size_t)blocks_for((size_t)N * tpp) * 8);
         a.epart = s.epart.p;
-        dim3 g(blocks_for((size_t)N * kTPP)), b(kBlock);
-        if (d_virial) hipLaunchKernelGGL((k_forces<kTPP, false, true>), g, b, 0, st, a);
-        else hipLaunchKernelGGL((k_forces<kTPP, false, false>), g, b, 0, st, a);
+        // split energies need the energy instantiation
+        launch_k_any(tpp, d_split_energy != nullptr, d_virial != nullptr, dim3(blocks_for((size_t)N * tpp)),
+                     dim3(kBlock), 0, st, a);
         uint32_t herr = 0;
         HIP_OK(hipMemcpyAsync(&herr, s.err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));

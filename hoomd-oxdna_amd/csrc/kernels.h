@@ -1,5 +1,6 @@
-// kernels.h — gfx950 kernels of the oxDNA engine: per-step frames, cell list,
-// neighbour list, the fused force kernel, energy reduction, Langevin integration.
+// kernels.h — argument blocks and device helpers of the oxDNA engine's kernels:
+// per-step frames, cell binning, in-kernel neighbour rows, the fused force kernel,
+// energy reduction, Langevin integration.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,7 +15,7 @@ enum { S_FENE = 0, S_BEXC = 1, S_STACK = 2, S_NEXC = 3, S_HB = 4, S_CR = 5, S_CX
 
 // Flags / counters shared by the step kernels (device memory, one per context).
 struct DevFlags {
-    uint32_t rebuild;        // neighbour list must be rebuilt this step
+    uint32_t rebuild;        // neighbour rows must be rebuilt this step
     uint32_t overflow;       // bit0: a cell overflowed, bit1: a neighbour row overflowed
     uint32_t max_cell;       // largest cell occupancy seen
     uint32_t max_neigh;      // largest neighbour count seen
@@ -38,14 +39,61 @@ __device__ __forceinline__ void quat_axes(double w, double x, double y, double z
     f2.x = (float)(2.0 * (x * z + y * w)); f2.y = (float)(2.0 * (y * z - x * w)); f2.z = (float)(1.0 - 2.0 * (x * x + y * y));
 }
 
+// Cell of a position in a box centred on the origin (cells of side >= r_list).
+__device__ __forceinline__ uint32_t cell_index(double x, double y, double z, const double* Linv, const int* nc) {
+    double u[3] = {x * Linv[0] + 0.5, y * Linv[1] + 0.5, z * Linv[2] + 0.5};
+    int c[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double v = u[a] - floor(u[a]);
+        int k = (int)(v * nc[a]);
+        c[a] = k >= nc[a] ? nc[a] - 1 : (k < 0 ? 0 : k);
+    }
+    return ((uint32_t)c[2] * nc[1] + c[1]) * nc[0] + c[0];
+}
+
+struct CellArgs {
+    uint32_t N;
+    const double4* pos;
+    double Linv[3];
+    int nc[3];
+    uint32_t cap;
+    uint32_t* count;    // NULL: no cell list (no nonbonded term)
+    uint32_t* members;
+    DevFlags* flags;
+};
+
+__device__ __forceinline__ void bin_particle(const CellArgs& a, uint32_t i, double x, double y, double z) {
+    uint32_t c = cell_index(x, y, z, a.Linv, a.nc);
+    uint32_t slot = atomicAdd(&a.count[c], 1u);
+    if (slot < a.cap) a.members[(size_t)c * a.cap + slot] = i;
+    else atomicOr(&a.flags->overflow, 1u);
+    atomicMax(&a.flags->max_cell, slot + 1);
+}
+
+// In-kernel neighbour-row construction (scan_cells in engine.hip).
+struct NbrArgs {
+    const double4* pos;
+    double L[3], Linv[3];
+    int nc[3];
+    uint32_t cap;
+    const uint32_t* count;
+    const uint32_t* members;
+    double rlist2;
+    uint32_t max_neigh;
+    double4* pos_last;
+    DevFlags* flags;       // NULL: never rebuild (rows supplied by the caller)
+    int force_rebuild;     // rebuild regardless of flags->rebuild
+};
+
 struct ForceArgs {
     uint32_t N;
     const double4* pos;    // x, y, z, type (low 32 bits of .w)
     const float4* fr0;     // a1, charge
     const float4* fr1;     // a2, type (int bits)
     const float4* fr2;     // a3, unused
-    const uint32_t* n_neigh;
-    const uint32_t* nlist;
+    uint32_t* n_neigh;
+    uint32_t* nlist;
     const size_t* head;
     const int* bpart;      // 2 per particle: partner or -1
     const int* binfo;      // 2 per particle: bit0 role (0: this is A, 1: this is B), bits 1.. bond type
@@ -58,10 +106,32 @@ struct ForceArgs {
     size_t vpitch;
     double* epart;         // per block 8 partial energies (WANT_E)
     int accumulate;
-    uint32_t* cell_count;  // zeroed here when non-null
-    uint32_t ncell_total;
-    double4* split;        // optional per-particle (fene, bexc, stack) half energies
+    double4* split;        // optional per-particle (fene, bexc, stack) half energies (WANT_E)
     unsigned long long* counts;  // COUNT instantiation: per term (inside cut-off, nonzero)
+    NbrArgs nb;
+};
+
+struct StepArgs {
+    uint32_t N;
+    double4* bd;       // Langevin force (xyz) of the last second half-kick, lab frame
+    double4* bd_tq;    // Langevin torque (xyz) of the last second half-kick, lab frame
+    double4* pos;
+    double4* vel;
+    double4* quat;
+    double4* angmom;
+    const float4* force;
+    const float4* torque;
+    const double* charge;
+    float4* fr0;
+    float4* fr1;
+    float4* fr2;
+    const double4* pos_last;
+    double dt, kT, gamma, gamma_r, mass, inertia;
+    double L[3], Linv[3];
+    double skin_half2;
+    unsigned long long seed;
+    DevFlags* flags;
+    CellArgs cells;
 };
 
 }  // namespace oxd
