@@ -23,6 +23,8 @@
 #include <string>
 #include <vector>
 
+#include "ensemble.h"
+#include "integrator.h"
 #include "kernels.h"
 #include "oxdna_hip.h"
 
@@ -199,61 +201,7 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
             const float rsq = dot(dr, dr);
             const V Rc = -dr;
             V Fp = zero3();
-            float ep = 0.f;
-            if ((mask & (1u << T_EXCVOL)) && rsq < P.rcutsq[T_EXCVOL][tp]) {
-                float e = ev_pair(Rc, fi, fj, P.ev, Fp, T);
-                et[S_NEXC] += e; ep += e;
-                if (COUNT) { cnt[2 * T_EXCVOL]++; cnt[2 * T_EXCVOL + 1] += e != 0.f; }
-            }
-            if ((mask & (1u << T_HBOND)) && rsq < P.rcutsq[T_HBOND][tp]) {
-                BB g;
-                if (bb_geom(Rc, fi, fj, P.hb.base, g)) {
-                    float eps = P.hb.hb_eps[tp], rv, rd;
-                    f1_vd(g.rmod, P.hb.f1, eps, eps * P.hb.shift_factor, rv, rd);
-                    float e = bb_term(g, fi, fj, rv, rd, P.hb.hb_t1, P.hb.hb_t23, P.hb.hb_t4, P.hb.hb_t78,
-                                      false, true, Fp, T);
-                    et[S_HB] += e; ep += e;
-                    if (COUNT) cnt[2 * T_HBOND + 1] += e != 0.f;
-                }
-                if (COUNT) cnt[2 * T_HBOND]++;
-            }
-            if ((mask & (1u << T_CROSS)) && rsq < P.rcutsq[T_CROSS][tp]) {
-                BB g;
-                if (bb_geom(Rc, fi, fj, P.cs.base, g)) {
-                    float rv, rd;
-                    f2_vd(g.rmod, P.cs.f2, rv, rd);
-                    float e = bb_term(g, fi, fj, rv, rd, P.cs.cs_t1, P.cs.cs_t23, P.cs.cs_t4, P.cs.cs_t78,
-                                      true, P.cs.cs_t4_enabled != 0, Fp, T);
-                    et[S_CR] += e; ep += e;
-                    if (COUNT) cnt[2 * T_CROSS + 1] += e != 0.f;
-                }
-                if (COUNT) cnt[2 * T_CROSS]++;
-            }
-            if ((mask & (1u << T_HBONDCROSS)) && rsq < P.rcutsq[T_HBONDCROSS][tp]) {
-                BB g;
-                if (bb_geom(Rc, fi, fj, P.hbx.base, g)) {
-                    float eps = P.hbx.hb_eps[tp], rv, rd;
-                    f1_vd(g.rmod, P.hbx.f1, eps, eps * P.hbx.shift_factor, rv, rd);
-                    float e1 = bb_term(g, fi, fj, rv, rd, P.hbx.hb_t1, P.hbx.hb_t23, P.hbx.hb_t4, P.hbx.hb_t78,
-                                       false, true, Fp, T);
-                    f2_vd(g.rmod, P.hbx.f2, rv, rd);
-                    float e2 = bb_term(g, fi, fj, rv, rd, P.hbx.cs_t1, P.hbx.cs_t23, P.hbx.cs_t4, P.hbx.cs_t78,
-                                       true, P.hbx.cs_t4_enabled != 0, Fp, T);
-                    et[S_HB] += e1; et[S_CR] += e2; ep += e1 + e2;
-                    if (COUNT) cnt[2 * T_HBONDCROSS + 1] += (e1 != 0.f || e2 != 0.f);
-                }
-                if (COUNT) cnt[2 * T_HBONDCROSS]++;
-            }
-            if ((mask & (1u << T_COAX)) && rsq < P.rcutsq[T_COAX][tp]) {
-                float e = coax_pair(Rc, fi, fj, P.cx, Fp, T);
-                et[S_CX] += e; ep += e;
-                if (COUNT) { cnt[2 * T_COAX]++; cnt[2 * T_COAX + 1] += e != 0.f; }
-            }
-            if ((mask & (1u << T_DEBYE)) && rsq < P.rcutsq[T_DEBYE][tp]) {
-                float e = dh_pair(Rc, fi, fj, qi * g0.w, P.dh, Fp, T);
-                et[S_DH] += e; ep += e;
-                if (COUNT) { cnt[2 * T_DEBYE]++; cnt[2 * T_DEBYE + 1] += e != 0.f; }
-            }
+            const float ep = pair_terms<COUNT>(P, mask, tp, rsq, Rc, fi, fj, qi * g0.w, Fp, T, et, cnt);
             F += Fp;
             Eh += 0.5f * ep;
             if (WANT_V) {
@@ -378,99 +326,16 @@ __global__ void k_reduce_energy(uint32_t nblocks, const double* __restrict__ par
 }
 
 // ------------------------------------------------------------------ integrator
-// Counter-based random numbers: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
-__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
-        uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
-        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
-        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
-    }
-}
-__device__ __forceinline__ float u01f(uint32_t x) { return ((float)(x >> 8) + 0.5f) * (1.0f / 16777216.0f); }
-
-// q1 (x) q2 for quaternions stored (w, x, y, z) in double4 (x=w, y=x, z=y, w=z).
-__device__ __forceinline__ double4 qmul(double4 a, double4 b) {
-    return make_double4(a.x * b.x - a.y * b.y - a.z * b.z - a.w * b.w,
-                        a.x * b.y + a.y * b.x + a.z * b.w - a.w * b.z,
-                        a.x * b.z - a.y * b.w + a.z * b.x + a.w * b.y,
-                        a.x * b.w + a.y * b.z - a.z * b.y + a.w * b.x);
-}
-__device__ __forceinline__ double4 qconj(double4 a) { return make_double4(a.x, -a.y, -a.z, -a.w); }
-// rotate the lab vector v into the body frame: conj(q) v q
-__device__ __forceinline__ void to_body(double4 q, double vx, double vy, double vz, double& bx, double& by, double& bz) {
-    double4 t = qmul(qmul(qconj(q), make_double4(0.0, vx, vy, vz)), q);
-    bx = t.y; by = t.z; bz = t.w;
-}
-__device__ __forceinline__ void to_lab(double4 q, double vx, double vy, double vz, double& bx, double& by, double& bz) {
-    double4 t = qmul(qmul(q, make_double4(0.0, vx, vy, vz)), qconj(q));
-    bx = t.y; by = t.z; bz = t.w;
-}
-// cos and sin of x: Taylor series through x^13 for |x| <= 1/4 (truncation below
-// 2^-60, i.e. exact in FP64), the library functions otherwise.
-__device__ __forceinline__ void sincos_small(double x, double& s, double& c) {
-    if (fabs(x) <= 0.25) {
-        double x2 = x * x;
-        s = x * (1.0 + x2 * (-1.0 / 6 + x2 * (1.0 / 120 + x2 * (-1.0 / 5040 + x2 * (1.0 / 362880 +
-                 x2 * (-1.0 / 39916800 + x2 * (1.0 / 6227020800.0)))))));
-        c = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24 + x2 * (-1.0 / 720 + x2 * (1.0 / 40320 + x2 * (-1.0 / 3628800 +
-                 x2 * (1.0 / 479001600.0))))));
-    } else {
-        sincos(x, &s, &c);
-    }
-}
-// Free-rotor sub-step about body axis k for time tau (Miller et al., J. Chem. Phys.
-// 116, 8649 (2002), NO_SQUISH): P_k permutes quaternion components.
-__device__ __forceinline__ double4 permute(int k, double4 q) {
-    if (k == 1) return make_double4(-q.y, q.x, q.w, -q.z);
-    if (k == 2) return make_double4(-q.z, -q.w, q.x, q.y);
-    return make_double4(-q.w, q.z, -q.y, q.x);
-}
-__device__ __forceinline__ void rotor(int k, double tau, double Ik, double4& q, double4& p) {
-    double4 Pq = permute(k, q), Pp = permute(k, p);
-    double zeta = (p.x * Pq.x + p.y * Pq.y + p.z * Pq.z + p.w * Pq.w) / (4.0 * Ik);
-    double s, c;
-    sincos_small(zeta * tau, s, c);
-    q = make_double4(c * q.x + s * Pq.x, c * q.y + s * Pq.y, c * q.z + s * Pq.z, c * q.w + s * Pq.w);
-    p = make_double4(c * p.x + s * Pp.x, c * p.y + s * Pp.y, c * p.z + s * Pp.z, c * p.w + s * Pp.w);
-}
-
-// First half of the step: v += dt/2 (F + F_L)/m, x += dt v (wrapped),
-// p += dt q (x) (tau + tau_L)_body, where F_L / tau_L are the Langevin force and torque
-// drawn in the previous second half (velocity Verlet on the total force, as the
-// host engine's Langevin method does), NO_SQUISH rotation (3: dt/2, 2: dt/2, 1: dt,
-// 2: dt/2, 3: dt/2), renormalised q; then the new frames, cell binning, and the skin
-// test that requests a neighbour-list rebuild.
+// First half of the step (lv_first_half), then the new frames, cell binning, and the
+// skin test that requests a neighbour-list rebuild.
 __global__ void k_step1(StepArgs a) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) a.flags->step += 1ull;
     if (i >= a.N) return;
-    const double dt = a.dt;
-    double4 x = a.pos[i], v = a.vel[i];
-    float4 f = a.force[i];
-    const double4 bd = a.bd[i], bdt = a.bd_tq[i];
-    const double hm = 0.5 * dt / a.mass;
-    v.x += hm * (f.x + bd.x); v.y += hm * (f.y + bd.y); v.z += hm * (f.z + bd.z);
-    x.x += dt * v.x; x.y += dt * v.y; x.z += dt * v.z;
-    x.x -= a.L[0] * floor(x.x * a.Linv[0] + 0.5);
-    x.y -= a.L[1] * floor(x.y * a.Linv[1] + 0.5);
-    x.z -= a.L[2] * floor(x.z * a.Linv[2] + 0.5);
-    double4 q = a.quat[i], p = a.angmom[i];
-    float4 t = a.torque[i];
-    double tbx, tby, tbz;
-    to_body(q, t.x + bdt.x, t.y + bdt.y, t.z + bdt.z, tbx, tby, tbz);
-    double4 kick = qmul(q, make_double4(0.0, tbx, tby, tbz));
-    p.x += dt * kick.x; p.y += dt * kick.y; p.z += dt * kick.z; p.w += dt * kick.w;
-    const double I = a.inertia;
-    rotor(3, 0.5 * dt, I, q, p);
-    rotor(2, 0.5 * dt, I, q, p);
-    rotor(1, dt, I, q, p);
-    rotor(2, 0.5 * dt, I, q, p);
-    rotor(3, 0.5 * dt, I, q, p);
-    double nq = 1.0 / sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
-    q.x *= nq; q.y *= nq; q.z *= nq; q.w *= nq;
+    const Langevin lv{a.dt, a.kT, a.gamma, a.gamma_r, a.mass, a.inertia};
+    double4 x = a.pos[i], v = a.vel[i], q = a.quat[i], p = a.angmom[i];
+    const float4 f = a.force[i], t = a.torque[i];
+    lv_first_half(lv, a.L, a.Linv, f.x, f.y, f.z, t.x, t.y, t.z, a.bd[i], a.bd_tq[i], x, v, q, p);
     a.pos[i] = x; a.vel[i] = v; a.quat[i] = q; a.angmom[i] = p;
     float4 f0, f1, f2;
     quat_axes(q.x, q.y, q.z, q.w, f0, f1, f2);
@@ -488,54 +353,19 @@ __global__ void k_step1(StepArgs a) {
     }
 }
 
-// Second half: Langevin forces (drag -gamma v plus uniform noise of variance
-// 2 gamma kT / dt per component; rotational drag -gamma_r L/I plus Gaussian noise of
-// variance 2 gamma_r kT / dt in the body frame), then v += dt/2 F/m, p += dt q (x) tau.
-// Also clears this particle's cell count for the next step's binning.
+// Second half (lv_second_half); also clears this particle's cell count for the next
+// step's binning.
 __global__ void k_step2(StepArgs a) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) a.flags->rebuild = 0u;
     if (i >= a.N) return;
-    const double dt = a.dt;
-    const unsigned long long step = a.flags->step;
-    uint32_t c[4] = {i, (uint32_t)step, (uint32_t)(step >> 32), 0u};
-    philox(c, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
-    uint32_t d[4] = {i, (uint32_t)step, (uint32_t)(step >> 32), 1u};
-    philox(d, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
-    double4 v = a.vel[i];
+    const Langevin lv{a.dt, a.kT, a.gamma, a.gamma_r, a.mass, a.inertia};
     const double4 x = a.pos[i];
     if (a.cells.count) a.cells.count[cell_index(x.x, x.y, x.z, a.cells.Linv, a.cells.nc)] = 0u;
-    float4 f = a.force[i];
-    const double ct = sqrt(6.0 * a.gamma * a.kT / dt);
-    double bx = -a.gamma * v.x + ct * (2.0 * u01f(c[0]) - 1.0);
-    double by = -a.gamma * v.y + ct * (2.0 * u01f(c[1]) - 1.0);
-    double bz = -a.gamma * v.z + ct * (2.0 * u01f(c[2]) - 1.0);
-    const double hm = 0.5 * dt / a.mass;
-    v.x += hm * (f.x + bx); v.y += hm * (f.y + by); v.z += hm * (f.z + bz);
-    a.vel[i] = v;
-    a.bd[i] = make_double4(bx, by, bz, 0.0);
-    double4 q = a.quat[i], p = a.angmom[i];
-    float4 t = a.torque[i];
-    double tbx, tby, tbz;
-    to_body(q, t.x, t.y, t.z, tbx, tby, tbz);
-    double4 Lb = qmul(qconj(q), p);  // = 2 (0, L_body)
-    const double I = a.inertia;
-    const double cr = sqrt(2.0 * a.gamma_r * a.kT / dt);
-    // Box-Muller in FP32: the noise needs no FP64 accuracy
-    float r1 = sqrtf(-2.f * __logf(u01f(c[3]))), r2 = sqrtf(-2.f * __logf(u01f(d[1])));
-    float s0, c0, s2, c2;
-    __sincosf(6.2831853f * u01f(d[0]), &s0, &c0);
-    __sincosf(6.2831853f * u01f(d[2]), &s2, &c2);
-    double rbx = cr * (double)(r1 * c0) - a.gamma_r * (0.5 * Lb.y) / I;
-    double rby = cr * (double)(r1 * s0) - a.gamma_r * (0.5 * Lb.z) / I;
-    double rbz = cr * (double)(r2 * c2) - a.gamma_r * (0.5 * Lb.w) / I;
-    tbx += rbx; tby += rby; tbz += rbz;
-    double lx, ly, lz;
-    to_lab(q, rbx, rby, rbz, lx, ly, lz);
-    a.bd_tq[i] = make_double4(lx, ly, lz, 0.0);
-    double4 kick = qmul(q, make_double4(0.0, tbx, tby, tbz));
-    p.x += dt * kick.x; p.y += dt * kick.y; p.z += dt * kick.z; p.w += dt * kick.w;
-    a.angmom[i] = p;
+    double4 v = a.vel[i], p = a.angmom[i], bd, bdt;
+    const float4 f = a.force[i], t = a.torque[i];
+    lv_second_half(lv, i, a.flags->step, a.seed, f.x, f.y, f.z, t.x, t.y, t.z, a.quat[i], v, p, bd, bdt);
+    a.vel[i] = v; a.angmom[i] = p; a.bd[i] = bd; a.bd_tq[i] = bdt;
 }
 
 }  // namespace oxd
@@ -629,6 +459,18 @@ struct oxdna_ctx {
     int graph_steps = 0;
     uint64_t host_step = 0;
 
+    // batched replicas (n_rep > 0): particles [rep_off[r], rep_off[r+1]) form replica r
+    uint32_t n_rep = 0;
+    uint32_t rep_nmax = 0;
+    std::vector<uint32_t> rep_off_h;
+    std::vector<double> rep_box_h;
+    DBuf<uint32_t> rep_off, rep_order;
+    DBuf<double> rep_box, rep_energy;
+    DBuf<EnsBias> rep_bias;
+    bool have_bias = false;
+    int ens_nt = 0;                      // workgroup size of the replica kernel (autotuned)
+    uint32_t ens_tuned_mask = 0xffffffffu;
+
     ~oxdna_ctx() {
         if (graph) (void)hipGraphExecDestroy(graph);
         pos.release(); vel.release(); quat.release(); angmom.release(); pos_last.release();
@@ -637,6 +479,7 @@ struct oxdna_ctx {
         virial.release(); bpart.release(); binfo.release(); n_neigh.release(); nlist.release();
         cell_count.release(); cell_members.release(); head.release(); epart.release(); energy.release();
         flags.release(); dparams.release();
+        rep_off.release(); rep_order.release(); rep_box.release(); rep_energy.release(); rep_bias.release();
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 
@@ -1182,6 +1025,138 @@ void run(oxdna_ctx* c, uint32_t nsteps, uint32_t mask, int use_graph) {
     c->host_step += nsteps;
 }
 
+// ------------------------------------------------------------------ batched replicas
+constexpr uint32_t kEnsMaxN = 512;     // largest replica (particles), one workgroup each
+constexpr uint32_t kEnsChunk = 2048;   // Langevin steps per persistent launch
+
+// Sums the per-replica energies into the context's 8 term totals (one block, fixed order).
+__global__ void k_rep_sum(uint32_t n_rep, const double* __restrict__ rep, double* out) {
+    __shared__ double s[256];
+    for (int slot = 0; slot < 8; ++slot) {
+        double v = 0.0;
+        for (uint32_t r = threadIdx.x; r < n_rep; r += blockDim.x) v += rep[(size_t)r * ENS_SLOTS + slot];
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[slot] = s[0];
+        __syncthreads();
+    }
+}
+
+void ensure_ready_ens(oxdna_ctx* c, uint32_t m) {
+    if (c->N == 0) throw Fail("no particles: call oxdna_set_state first");
+    upload_params(c);
+    double rmax = pair_rmax(c, m);
+    for (uint32_t r = 0; r < c->n_rep; ++r)
+        for (int k = 0; k < 3; ++k)
+            if (!(c->rep_box_h[3 * r + k] > 2.0 * rmax))
+                throw Fail("replica " + std::to_string(r) + ": box length must exceed twice the largest cut-off");
+}
+
+EnsArgs ens_args(oxdna_ctx* c, uint32_t mask, uint32_t nsteps, uint32_t nevals, int want_e) {
+    EnsArgs a;
+    a.n_rep = c->n_rep;
+    a.offsets = c->rep_off.p;
+    a.order = c->rep_order.p;
+    a.boxes = c->rep_box.p;
+    a.pos = c->pos.p; a.vel = c->vel.p; a.quat = c->quat.p; a.angmom = c->angmom.p;
+    a.bd = c->bd.p; a.bd_tq = c->bd_tq.p;
+    a.charge = c->charge.p;
+    a.bpart = c->bpart.p; a.binfo = c->binfo.p;
+    a.P = c->dparams.p;
+    a.mask = mask;
+    a.force = c->force.p; a.torque = c->torque.p;
+    a.rep_energy = c->rep_energy.p;
+    a.bias = c->have_bias ? c->rep_bias.p : nullptr;
+    a.lv = Langevin{c->dt, c->kT, c->gamma, c->gamma_r, 1.0, 1.0};
+    a.seed = c->seed;
+    a.step0 = c->host_step;
+    a.nsteps = nsteps;
+    a.nevals = nevals;
+    a.want_e = want_e;
+    a.counts = nullptr;
+    return a;
+}
+
+void launch_ens(oxdna_ctx* c, int nt, bool count, const EnsArgs& a) {
+    if (nt < 64 || (uint32_t)nt < c->rep_nmax || a.n_rep == 0 || !a.offsets || !a.order || !a.P)
+        throw Fail("internal: replica kernel launch does not fit the replicas");
+    dim3 g(a.n_rep);
+    if (count) {
+        hipLaunchKernelGGL((k_ensemble<512, true>), g, dim3(512), 0, c->stream, a);
+        return;
+    }
+    switch (nt) {
+    case 64: hipLaunchKernelGGL((k_ensemble<64, false>), g, dim3(64), 0, c->stream, a); break;
+    case 128: hipLaunchKernelGGL((k_ensemble<128, false>), g, dim3(128), 0, c->stream, a); break;
+    case 256: hipLaunchKernelGGL((k_ensemble<256, false>), g, dim3(256), 0, c->stream, a); break;
+    case 512: hipLaunchKernelGGL((k_ensemble<512, false>), g, dim3(512), 0, c->stream, a); break;
+    default: throw Fail("internal: unsupported replica workgroup size");
+    }
+}
+
+int ens_default_nt(const oxdna_ctx* c) {
+    int nt = 64;
+    while ((uint32_t)nt < c->rep_nmax) nt *= 2;
+    return nt;
+}
+
+// Autotuner: workgroup size of the replica kernel (the smallest that holds the
+// largest replica, and each doubling up to 512), timed as forces-only launches
+// (the state is not changed) with HIP events; keeps the fastest.
+void autotune_ens(oxdna_ctx* c, uint32_t m) {
+    if (c->ens_tuned_mask == m && c->ens_nt != 0) return;
+    float best = 1e30f;
+    int best_nt = ens_default_nt(c);
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    for (int nt = ens_default_nt(c); nt <= (int)kEnsMaxN; nt *= 2) {
+        EnsArgs a = ens_args(c, m, 0, 8, 0);
+        launch_ens(c, nt, false, a);
+        HIP_OK(hipEventRecord(e0, c->stream));
+        for (int rep = 0; rep < 3; ++rep) launch_ens(c, nt, false, a);
+        HIP_OK(hipEventRecord(e1, c->stream));
+        HIP_OK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; best_nt = nt; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->ens_nt = best_nt;
+    c->ens_tuned_mask = m;
+}
+
+void compute_ens(oxdna_ctx* c, uint32_t mask) {
+    uint32_t m = effective_mask(c, mask);
+    ensure_ready_ens(c, m);
+    int nt = c->ens_nt ? c->ens_nt : ens_default_nt(c);
+    launch_ens(c, nt, false, ens_args(c, m, 0, 1, 1));
+    hipLaunchKernelGGL(k_rep_sum, dim3(1), dim3(256), 0, c->stream, c->n_rep, c->rep_energy.p, c->energy.p);
+    HIP_OK(hipMemsetAsync(c->virial.p, 0, sizeof(float) * 6 * (size_t)c->N, c->stream));
+    c->forces_mask = m;
+}
+
+void run_ens(oxdna_ctx* c, uint32_t nsteps, uint32_t mask) {
+    if (!c->have_langevin) throw Fail("no integrator: call oxdna_set_langevin first");
+    uint32_t m = effective_mask(c, mask);
+    ensure_ready_ens(c, m);
+    if (c->forces_mask != m) compute_ens(c, m);
+    autotune_ens(c, m);
+    uint32_t done = 0;
+    while (done < nsteps) {
+        uint32_t chunk = std::min(kEnsChunk, nsteps - done);
+        launch_ens(c, c->ens_nt, false, ens_args(c, m, chunk, 0, 0));
+        c->host_step += chunk;
+        done += chunk;
+    }
+    HIP_OK(hipGetLastError());
+}
+
 }  // namespace
 
 // =====================================================================================
@@ -1267,6 +1242,8 @@ int oxdna_set_state(oxdna_ctx* c, unsigned int n, const double* pos, const doubl
             c->torque.release(); c->virial.release(); c->n_neigh.release(); c->nlist.release(); c->head.release();
             c->epart.release();
             c->have_bonds = false;
+            c->n_rep = 0;  // the replica layout no longer fits
+            c->have_bias = false;
         }
         c->pos.alloc(n); c->quat.alloc(n); c->vel.alloc(n); c->angmom.alloc(n); c->pos_last.alloc(n);
         c->bd.alloc(n); c->bd_tq.alloc(n);
@@ -1407,7 +1384,10 @@ int oxdna_set_nlist(oxdna_ctx* c, double buffer) {
 
 int oxdna_compute(oxdna_ctx* c, unsigned int mask, int fused) {
     if (!c) return fail(nullptr, "NULL context");
-    return guard(c, [&] { compute(c, mask, fused); });
+    return guard(c, [&] {
+        if (c->n_rep) compute_ens(c, mask);
+        else compute(c, mask, fused);
+    });
 }
 
 int oxdna_get_term_energies(oxdna_ctx* c, double out[8]) {
@@ -1441,6 +1421,7 @@ int oxdna_get_nlist_pairs(oxdna_ctx* c, int32_t* pairs, size_t max_pairs, size_t
     if (!c) return fail(nullptr, "NULL context");
     int rc = 0;
     int g = guard(c, [&] {
+        if (c->n_rep) throw Fail("oxdna_get_nlist_pairs: replicas use no neighbour list");
         if (!c->nlist_valid) compute(c, 0x3fu, 1);
         uint32_t n = c->N;
         std::vector<uint32_t> nn(n), rows((size_t)n * c->max_neigh);
@@ -1478,7 +1459,8 @@ int oxdna_run(oxdna_ctx* c, unsigned int nsteps, unsigned int mask, int use_grap
     if (!c) return fail(nullptr, "NULL context");
     return guard(c, [&] {
         if (nsteps == 0) return;
-        run(c, nsteps, mask, use_graph);
+        if (c->n_rep) run_ens(c, nsteps, mask);
+        else run(c, nsteps, mask, use_graph);
     });
 }
 
@@ -1502,28 +1484,153 @@ int oxdna_get_state(oxdna_ctx* c, double* pos, double* quat, double* vel, double
 }
 
 
+int oxdna_set_replicas(oxdna_ctx* c, unsigned int n_replicas, const unsigned int* offsets, const double* boxes) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (n_replicas == 0) {
+            c->n_rep = 0;
+            c->have_bias = false;
+            c->forces_mask = 0xffffffffu;
+            return;
+        }
+        if (c->N == 0) throw Fail("oxdna_set_replicas: set the state first");
+        if (!offsets || !boxes) throw Fail("oxdna_set_replicas: NULL array");
+        if (offsets[0] != 0 || offsets[n_replicas] != c->N)
+            throw Fail("oxdna_set_replicas: offsets must run from 0 to the particle count");
+        uint32_t nmax = 0;
+        for (uint32_t r = 0; r < n_replicas; ++r) {
+            if (offsets[r + 1] <= offsets[r]) throw Fail("oxdna_set_replicas: every replica needs a particle");
+            nmax = std::max<uint32_t>(nmax, offsets[r + 1] - offsets[r]);
+            for (int k = 0; k < 3; ++k)
+                if (!(boxes[3 * r + k] > 0.0)) throw Fail("oxdna_set_replicas: box lengths must be positive");
+        }
+        if (nmax > kEnsMaxN)
+            throw Fail("oxdna_set_replicas: a replica has " + std::to_string(nmax) + " particles; at most " +
+                       std::to_string(kEnsMaxN) + " (use one context per larger system)");
+        // bonds must stay inside their replica
+        std::vector<int> part(2 * (size_t)c->N);
+        HIP_OK(hipMemcpyAsync(part.data(), c->bpart.p, sizeof(int) * part.size(), hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        for (uint32_t r = 0; r < n_replicas; ++r)
+            for (uint32_t i = offsets[r]; i < offsets[r + 1]; ++i)
+                for (int s = 0; s < 2; ++s) {
+                    int o = part[2 * (size_t)i + s];
+                    if (o >= 0 && ((uint32_t)o < offsets[r] || (uint32_t)o >= offsets[r + 1]))
+                        throw Fail("oxdna_set_replicas: bond " + std::to_string(i) + "-" + std::to_string(o) +
+                                   " crosses replicas");
+                }
+        c->rep_off_h.assign(offsets, offsets + n_replicas + 1);
+        c->rep_box_h.assign(boxes, boxes + 3 * (size_t)n_replicas);
+        // largest replicas first: the workgroups that take longest start first
+        std::vector<uint32_t> order(n_replicas);
+        for (uint32_t r = 0; r < n_replicas; ++r) order[r] = r;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            return offsets[x + 1] - offsets[x] > offsets[y + 1] - offsets[y];
+        });
+        c->rep_off.release(); c->rep_order.release(); c->rep_box.release(); c->rep_energy.release();
+        c->rep_bias.release();
+        c->rep_off.alloc(n_replicas + 1);
+        c->rep_order.alloc(n_replicas);
+        c->rep_box.alloc(3 * (size_t)n_replicas);
+        c->rep_energy.alloc(ENS_SLOTS * (size_t)n_replicas);
+        HIP_OK(hipMemcpyAsync(c->rep_off.p, offsets, sizeof(uint32_t) * (n_replicas + 1), hipMemcpyHostToDevice,
+                              c->stream));
+        HIP_OK(hipMemcpyAsync(c->rep_order.p, order.data(), sizeof(uint32_t) * n_replicas, hipMemcpyHostToDevice,
+                              c->stream));
+        HIP_OK(hipMemcpyAsync(c->rep_box.p, boxes, sizeof(double) * 3 * n_replicas, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemsetAsync(c->rep_energy.p, 0, sizeof(double) * ENS_SLOTS * n_replicas, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        c->n_rep = n_replicas;
+        c->rep_nmax = nmax;
+        c->have_bias = false;
+        c->ens_nt = 0;
+        c->ens_tuned_mask = 0xffffffffu;
+        c->forces_mask = 0xffffffffu;
+        c->drop_graph();
+    });
+}
+
+int oxdna_set_replica_bias(oxdna_ctx* c, const int32_t* ends, const double* k, const double* d0) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (c->n_rep == 0) throw Fail("oxdna_set_replica_bias: set the replicas first");
+        if (!ends || !k || !d0) {
+            c->have_bias = false;
+            c->forces_mask = 0xffffffffu;
+            return;
+        }
+        std::vector<EnsBias> b(c->n_rep);
+        for (uint32_t r = 0; r < c->n_rep; ++r) {
+            int n = (int)(c->rep_off_h[r + 1] - c->rep_off_h[r]);
+            if (ends[2 * r] < 0 || ends[2 * r] >= n || ends[2 * r + 1] < 0 || ends[2 * r + 1] >= n ||
+                ends[2 * r] == ends[2 * r + 1])
+                throw Fail("oxdna_set_replica_bias: replica " + std::to_string(r) + " has invalid end indices");
+            if (!(k[r] >= 0.0) || !std::isfinite(d0[r]))
+                throw Fail("oxdna_set_replica_bias: need k >= 0 and a finite d0");
+            b[r] = EnsBias{k[r], d0[r], ends[2 * r], ends[2 * r + 1]};
+        }
+        c->rep_bias.alloc(c->n_rep);
+        HIP_OK(hipMemcpyAsync(c->rep_bias.p, b.data(), sizeof(EnsBias) * c->n_rep, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        c->have_bias = true;
+        c->forces_mask = 0xffffffffu;
+    });
+}
+
+int oxdna_get_replica_energies(oxdna_ctx* c, double* out) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (c->n_rep == 0) throw Fail("oxdna_get_replica_energies: no replicas");
+        if (!out) throw Fail("oxdna_get_replica_energies: out is NULL");
+        HIP_OK(hipMemcpyAsync(out, c->rep_energy.p, sizeof(double) * ENS_SLOTS * c->n_rep, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    });
+}
+
 int oxdna_time_forces(oxdna_ctx* c, unsigned int mask, unsigned int nrep, double* avg_ms) {
     if (!c) return fail(nullptr, "NULL context");
     return guard(c, [&] {
         if (nrep == 0) throw Fail("oxdna_time_forces: nrep must be > 0");
+        if (c->n_rep) {
+            // replicas: one forces-only launch evaluating nrep times
+            uint32_t m = effective_mask(c, mask);
+            ensure_ready_ens(c, m);
+            autotune_ens(c, m);
+            hipEvent_t e0, e1;
+            HIP_OK(hipEventCreate(&e0));
+            HIP_OK(hipEventCreate(&e1));
+            EnsArgs a = ens_args(c, m, 0, nrep, 0);
+            launch_ens(c, c->ens_nt, false, ens_args(c, m, 0, 1, 0));
+            HIP_OK(hipEventRecord(e0, c->stream));
+            launch_ens(c, c->ens_nt, false, a);
+            HIP_OK(hipEventRecord(e1, c->stream));
+            HIP_OK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            *avg_ms = (double)ms / nrep;
+            return;
+        }
         ensure_ready(c);
         uint32_t m = effective_mask(c, mask);
         if (c->forces_mask != m || !c->nlist_valid) compute(c, m, 1);
-        std::vector<hipEvent_t> ev(2 * nrep);
-        for (auto& x : ev) HIP_OK(hipEventCreate(&x));
-        for (uint32_t r = 0; r < nrep; ++r) {
-            HIP_OK(hipEventRecord(ev[2 * r], c->stream));
-            launch_forces(c, m, 0, false, false, false);
-            HIP_OK(hipEventRecord(ev[2 * r + 1], c->stream));
-        }
-        HIP_OK(hipStreamSynchronize(c->stream));
-        double tot = 0.0;
-        for (uint32_t r = 0; r < nrep; ++r) {
-            float ms = 0.f;
-            HIP_OK(hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]));
-            tot += ms;
-        }
-        for (auto& x : ev) (void)hipEventDestroy(x);
+        // one event pair around nrep back-to-back launches on the context's stream: the
+        // average then holds no per-launch event overhead, as inside the step graph
+        hipEvent_t e0, e1;
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        launch_forces(c, m, 0, false, false, false);  // warm caches and clocks
+        HIP_OK(hipEventRecord(e0, c->stream));
+        for (uint32_t r = 0; r < nrep; ++r) launch_forces(c, m, 0, false, false, false);
+        HIP_OK(hipEventRecord(e1, c->stream));
+        HIP_OK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        double tot = ms;
         *avg_ms = tot / nrep;
     });
 }
@@ -1531,6 +1638,23 @@ int oxdna_time_forces(oxdna_ctx* c, unsigned int mask, unsigned int nrep, double
 int oxdna_count_evaluations(oxdna_ctx* c, unsigned int mask, uint64_t out[14]) {
     if (!c) return fail(nullptr, "NULL context");
     return guard(c, [&] {
+        if (c->n_rep) {
+            // replicas: every unordered pair and every bond once
+            uint32_t m = effective_mask(c, mask);
+            ensure_ready_ens(c, m);
+            DBuf<unsigned long long> d;
+            d.alloc(14);
+            HIP_OK(hipMemsetAsync(d.p, 0, 14 * sizeof(unsigned long long), c->stream));
+            EnsArgs a = ens_args(c, m, 0, 1, 0);
+            a.counts = d.p;
+            launch_ens(c, 512, true, a);
+            unsigned long long h[14];
+            HIP_OK(hipMemcpyAsync(h, d.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+            HIP_OK(hipStreamSynchronize(c->stream));
+            d.release();
+            for (int k = 0; k < 14; ++k) out[k] = h[k];
+            return;
+        }
         ensure_ready(c);
         uint32_t m = effective_mask(c, mask);
         if (c->forces_mask != m || !c->nlist_valid) compute(c, m, 1);
@@ -1560,7 +1684,7 @@ int oxdna_get_stats(oxdna_ctx* c, uint64_t* n_builds, uint32_t* max_neigh, uint6
         read_flags(c, f);
         if (n_builds) *n_builds = f.builds;
         if (max_neigh) *max_neigh = c->max_neigh;
-        if (step) *step = f.step;
+        if (step) *step = c->n_rep ? c->host_step : f.step;
     });
 }
 

@@ -1,0 +1,150 @@
+// integrator.h — Langevin dynamics with NO_SQUISH rigid-body rotation, as device
+// functions shared by the one-system step kernels and the batched replica kernel, so
+// both advance a particle with the same arithmetic and the same random stream.
+//
+// Semantics follow the host engine's Langevin method (velocity Verlet on the total
+// force): the Langevin force and torque drawn in one step's second half are added to
+// the conservative force again in the next step's first half.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace oxd {
+
+// Counter-based random numbers: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+        uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+}
+__device__ __forceinline__ float u01f(uint32_t x) { return ((float)(x >> 8) + 0.5f) * (1.0f / 16777216.0f); }
+
+// q1 (x) q2 for quaternions stored (w, x, y, z) in double4 (x=w, y=x, z=y, w=z).
+__device__ __forceinline__ double4 qmul(double4 a, double4 b) {
+    return make_double4(a.x * b.x - a.y * b.y - a.z * b.z - a.w * b.w,
+                        a.x * b.y + a.y * b.x + a.z * b.w - a.w * b.z,
+                        a.x * b.z - a.y * b.w + a.z * b.x + a.w * b.y,
+                        a.x * b.w + a.y * b.z - a.z * b.y + a.w * b.x);
+}
+__device__ __forceinline__ double4 qconj(double4 a) { return make_double4(a.x, -a.y, -a.z, -a.w); }
+// rotate the lab vector v into the body frame: conj(q) v q
+__device__ __forceinline__ void to_body(double4 q, double vx, double vy, double vz, double& bx, double& by, double& bz) {
+    double4 t = qmul(qmul(qconj(q), make_double4(0.0, vx, vy, vz)), q);
+    bx = t.y; by = t.z; bz = t.w;
+}
+__device__ __forceinline__ void to_lab(double4 q, double vx, double vy, double vz, double& bx, double& by, double& bz) {
+    double4 t = qmul(qmul(q, make_double4(0.0, vx, vy, vz)), qconj(q));
+    bx = t.y; by = t.z; bz = t.w;
+}
+// cos and sin of x: Taylor series through x^13 for |x| <= 1/4 (truncation below
+// 2^-60, i.e. exact in FP64), the library functions otherwise.
+__device__ __forceinline__ void sincos_small(double x, double& s, double& c) {
+    if (fabs(x) <= 0.25) {
+        double x2 = x * x;
+        s = x * (1.0 + x2 * (-1.0 / 6 + x2 * (1.0 / 120 + x2 * (-1.0 / 5040 + x2 * (1.0 / 362880 +
+                 x2 * (-1.0 / 39916800 + x2 * (1.0 / 6227020800.0)))))));
+        c = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24 + x2 * (-1.0 / 720 + x2 * (1.0 / 40320 + x2 * (-1.0 / 3628800 +
+                 x2 * (1.0 / 479001600.0))))));
+    } else {
+        sincos(x, &s, &c);
+    }
+}
+// Free-rotor sub-step about body axis k for time tau (Miller et al., J. Chem. Phys.
+// 116, 8649 (2002), NO_SQUISH): P_k permutes quaternion components.
+__device__ __forceinline__ double4 permute(int k, double4 q) {
+    if (k == 1) return make_double4(-q.y, q.x, q.w, -q.z);
+    if (k == 2) return make_double4(-q.z, -q.w, q.x, q.y);
+    return make_double4(-q.w, q.z, -q.y, q.x);
+}
+__device__ __forceinline__ void rotor(int k, double tau, double Ik, double4& q, double4& p) {
+    double4 Pq = permute(k, q), Pp = permute(k, p);
+    double zeta = (p.x * Pq.x + p.y * Pq.y + p.z * Pq.z + p.w * Pq.w) / (4.0 * Ik);
+    double s, c;
+    sincos_small(zeta * tau, s, c);
+    q = make_double4(c * q.x + s * Pq.x, c * q.y + s * Pq.y, c * q.z + s * Pq.z, c * q.w + s * Pq.w);
+    p = make_double4(c * p.x + s * Pp.x, c * p.y + s * Pp.y, c * p.z + s * Pp.z, c * p.w + s * Pp.w);
+}
+
+struct Langevin {
+    double dt, kT, gamma, gamma_r, mass, inertia;
+};
+
+// First half of a step for one particle: v += dt/2 (F + F_L)/m, x += dt v (wrapped
+// into the box centred on the origin), p += dt q (x) (tau + tau_L)_body, NO_SQUISH
+// rotation (axis 3: dt/2, 2: dt/2, 1: dt, 2: dt/2, 3: dt/2), renormalised q.
+// (f, t): conservative force and torque (lab frame); (bd, bdt): the Langevin force and
+// torque drawn in the previous second half. x.w (type bits) is left untouched.
+__device__ __forceinline__ void lv_first_half(const Langevin& lv, const double L[3], const double Linv[3],
+                                              float fx, float fy, float fz, float tx, float ty, float tz,
+                                              const double4& bd, const double4& bdt,
+                                              double4& x, double4& v, double4& q, double4& p) {
+    const double dt = lv.dt;
+    const double hm = 0.5 * dt / lv.mass;
+    v.x += hm * (fx + bd.x); v.y += hm * (fy + bd.y); v.z += hm * (fz + bd.z);
+    x.x += dt * v.x; x.y += dt * v.y; x.z += dt * v.z;
+    x.x -= L[0] * floor(x.x * Linv[0] + 0.5);
+    x.y -= L[1] * floor(x.y * Linv[1] + 0.5);
+    x.z -= L[2] * floor(x.z * Linv[2] + 0.5);
+    double tbx, tby, tbz;
+    to_body(q, tx + bdt.x, ty + bdt.y, tz + bdt.z, tbx, tby, tbz);
+    double4 kick = qmul(q, make_double4(0.0, tbx, tby, tbz));
+    p.x += dt * kick.x; p.y += dt * kick.y; p.z += dt * kick.z; p.w += dt * kick.w;
+    const double I = lv.inertia;
+    rotor(3, 0.5 * dt, I, q, p);
+    rotor(2, 0.5 * dt, I, q, p);
+    rotor(1, dt, I, q, p);
+    rotor(2, 0.5 * dt, I, q, p);
+    rotor(3, 0.5 * dt, I, q, p);
+    double nq = 1.0 / sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    q.x *= nq; q.y *= nq; q.z *= nq; q.w *= nq;
+}
+
+// Second half for particle `i` at integration step `step` with random key `seed`:
+// Langevin force (drag -gamma v plus uniform noise of variance 2 gamma kT / dt per
+// component) and torque (drag -gamma_r L / I plus Gaussian noise of variance
+// 2 gamma_r kT / dt, body frame), then v += dt/2 (F + F_L)/m, p += dt q (x) (tau + tau_L).
+// Returns the Langevin force and torque (lab frame) in bd / bdt for the next step.
+__device__ __forceinline__ void lv_second_half(const Langevin& lv, uint32_t i, unsigned long long step,
+                                               unsigned long long seed, float fx, float fy, float fz,
+                                               float tx, float ty, float tz, const double4& q,
+                                               double4& v, double4& p, double4& bd, double4& bdt) {
+    const double dt = lv.dt;
+    uint32_t c[4] = {i, (uint32_t)step, (uint32_t)(step >> 32), 0u};
+    philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint32_t d[4] = {i, (uint32_t)step, (uint32_t)(step >> 32), 1u};
+    philox(d, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double ct = sqrt(6.0 * lv.gamma * lv.kT / dt);
+    double bx = -lv.gamma * v.x + ct * (2.0 * u01f(c[0]) - 1.0);
+    double by = -lv.gamma * v.y + ct * (2.0 * u01f(c[1]) - 1.0);
+    double bz = -lv.gamma * v.z + ct * (2.0 * u01f(c[2]) - 1.0);
+    const double hm = 0.5 * dt / lv.mass;
+    v.x += hm * (fx + bx); v.y += hm * (fy + by); v.z += hm * (fz + bz);
+    bd = make_double4(bx, by, bz, 0.0);
+    double tbx, tby, tbz;
+    to_body(q, tx, ty, tz, tbx, tby, tbz);
+    double4 Lb = qmul(qconj(q), p);  // = 2 (0, L_body)
+    const double I = lv.inertia;
+    const double cr = sqrt(2.0 * lv.gamma_r * lv.kT / dt);
+    // Box-Muller in FP32: the noise needs no FP64 accuracy
+    float r1 = sqrtf(-2.f * __logf(u01f(c[3]))), r2 = sqrtf(-2.f * __logf(u01f(d[1])));
+    float s0, c0, s2, c2;
+    __sincosf(6.2831853f * u01f(d[0]), &s0, &c0);
+    __sincosf(6.2831853f * u01f(d[2]), &s2, &c2);
+    double rbx = cr * (double)(r1 * c0) - lv.gamma_r * (0.5 * Lb.y) / I;
+    double rby = cr * (double)(r1 * s0) - lv.gamma_r * (0.5 * Lb.z) / I;
+    double rbz = cr * (double)(r2 * c2) - lv.gamma_r * (0.5 * Lb.w) / I;
+    tbx += rbx; tby += rby; tbz += rbz;
+    double lx, ly, lz;
+    to_lab(q, rbx, rby, rbz, lx, ly, lz);
+    bdt = make_double4(lx, ly, lz, 0.0);
+    double4 kick = qmul(q, make_double4(0.0, tbx, tby, tbz));
+    p.x += dt * kick.x; p.y += dt * kick.y; p.z += dt * kick.z; p.w += dt * kick.w;
+}
+
+}  // namespace oxd

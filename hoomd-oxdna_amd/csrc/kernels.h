@@ -52,6 +52,73 @@ __device__ __forceinline__ uint32_t cell_index(double x, double y, double z, con
     return ((uint32_t)c[2] * nc[1] + c[1]) * nc[0] + c[0];
 }
 
+// Every selected nonbonded term of one pair, i half: adds the force on i to Fp and
+// the torque on i to T, the per-term energies to et (split_energy slots) and returns
+// the pair energy. Rc = r_j - r_i (minimum image), rsq = |Rc|^2, tp the type-pair
+// index. With COUNT, cnt[2 t] counts evaluations of term t inside its cut-off and
+// cnt[2 t + 1] those with a nonzero energy.
+template<bool COUNT>
+__device__ __forceinline__ float pair_terms(const DParams& P, uint32_t mask, int tp, float rsq, V Rc,
+                                            const Frame& fi, const Frame& fj, float qiqj, V& Fp, V& T,
+                                            float et[8], uint32_t cnt[14]) {
+    float ep = 0.f;
+    if ((mask & (1u << T_EXCVOL)) && rsq < P.rcutsq[T_EXCVOL][tp]) {
+        float e = ev_pair(Rc, fi, fj, P.ev, Fp, T);
+        et[S_NEXC] += e; ep += e;
+        if (COUNT) { cnt[2 * T_EXCVOL]++; cnt[2 * T_EXCVOL + 1] += e != 0.f; }
+    }
+    if ((mask & (1u << T_HBOND)) && rsq < P.rcutsq[T_HBOND][tp]) {
+        BB g;
+        if (bb_geom(Rc, fi, fj, P.hb.base, g)) {
+            float eps = P.hb.hb_eps[tp], rv, rd;
+            f1_vd(g.rmod, P.hb.f1, eps, eps * P.hb.shift_factor, rv, rd);
+            float e = bb_term(g, fi, fj, rv, rd, P.hb.hb_t1, P.hb.hb_t23, P.hb.hb_t4, P.hb.hb_t78,
+                              false, true, Fp, T);
+            et[S_HB] += e; ep += e;
+            if (COUNT) cnt[2 * T_HBOND + 1] += e != 0.f;
+        }
+        if (COUNT) cnt[2 * T_HBOND]++;
+    }
+    if ((mask & (1u << T_CROSS)) && rsq < P.rcutsq[T_CROSS][tp]) {
+        BB g;
+        if (bb_geom(Rc, fi, fj, P.cs.base, g)) {
+            float rv, rd;
+            f2_vd(g.rmod, P.cs.f2, rv, rd);
+            float e = bb_term(g, fi, fj, rv, rd, P.cs.cs_t1, P.cs.cs_t23, P.cs.cs_t4, P.cs.cs_t78,
+                              true, P.cs.cs_t4_enabled != 0, Fp, T);
+            et[S_CR] += e; ep += e;
+            if (COUNT) cnt[2 * T_CROSS + 1] += e != 0.f;
+        }
+        if (COUNT) cnt[2 * T_CROSS]++;
+    }
+    if ((mask & (1u << T_HBONDCROSS)) && rsq < P.rcutsq[T_HBONDCROSS][tp]) {
+        BB g;
+        if (bb_geom(Rc, fi, fj, P.hbx.base, g)) {
+            float eps = P.hbx.hb_eps[tp], rv, rd;
+            f1_vd(g.rmod, P.hbx.f1, eps, eps * P.hbx.shift_factor, rv, rd);
+            float e1 = bb_term(g, fi, fj, rv, rd, P.hbx.hb_t1, P.hbx.hb_t23, P.hbx.hb_t4, P.hbx.hb_t78,
+                               false, true, Fp, T);
+            f2_vd(g.rmod, P.hbx.f2, rv, rd);
+            float e2 = bb_term(g, fi, fj, rv, rd, P.hbx.cs_t1, P.hbx.cs_t23, P.hbx.cs_t4, P.hbx.cs_t78,
+                               true, P.hbx.cs_t4_enabled != 0, Fp, T);
+            et[S_HB] += e1; et[S_CR] += e2; ep += e1 + e2;
+            if (COUNT) cnt[2 * T_HBONDCROSS + 1] += (e1 != 0.f || e2 != 0.f);
+        }
+        if (COUNT) cnt[2 * T_HBONDCROSS]++;
+    }
+    if ((mask & (1u << T_COAX)) && rsq < P.rcutsq[T_COAX][tp]) {
+        float e = coax_pair(Rc, fi, fj, P.cx, Fp, T);
+        et[S_CX] += e; ep += e;
+        if (COUNT) { cnt[2 * T_COAX]++; cnt[2 * T_COAX + 1] += e != 0.f; }
+    }
+    if ((mask & (1u << T_DEBYE)) && rsq < P.rcutsq[T_DEBYE][tp]) {
+        float e = dh_pair(Rc, fi, fj, qiqj, P.dh, Fp, T);
+        et[S_DH] += e; ep += e;
+        if (COUNT) { cnt[2 * T_DEBYE]++; cnt[2 * T_DEBYE + 1] += e != 0.f; }
+    }
+    return ep;
+}
+
 struct CellArgs {
     uint32_t N;
     const double4* pos;

@@ -14,7 +14,8 @@ from . import forces  # noqa: F401
 from . import sim  # noqa: F401
 from . import model  # noqa: F401
 from . import systems  # noqa: F401
+from . import ensemble  # noqa: F401
 
 version = "0.1.0"
 
-__all__ = ["io", "forces", "sim", "model", "systems", "version"]
+__all__ = ["io", "forces", "sim", "model", "systems", "ensemble", "version"]

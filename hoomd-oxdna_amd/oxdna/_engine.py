@@ -42,6 +42,9 @@ _SIGS = {
     "oxdna_get_stats": (C.c_int, [_p, C.POINTER(_u64), C.POINTER(_u32), C.POINTER(_u64)]),
     "oxdna_time_forces": (C.c_int, [_p, _u32, _u32, C.POINTER(C.c_double)]),
     "oxdna_count_evaluations": (C.c_int, [_p, _u32, C.POINTER(_u64)]),
+    "oxdna_set_replicas": (C.c_int, [_p, _u32, C.POINTER(_u32), _dp]),
+    "oxdna_set_replica_bias": (C.c_int, [_p, _ip, _dp, _dp]),
+    "oxdna_get_replica_energies": (C.c_int, [_p, _dp]),
     "oxdna_pair_forces": (C.c_int, [C.c_int, _p, _p, _p, C.c_size_t, _u32, _p, _p, _p, C.POINTER(_abi.Box),
                                     _p, _p, _p, C.c_int, _p, _dp, _p, _p]),
     "oxdna_bonded_forces": (C.c_int, [_p, _p, _p, C.c_size_t, _p, _u32, _p, _p, C.POINTER(_abi.Box), _p, _p,

@@ -175,60 +175,92 @@ def duplex_gas(n_duplexes, n_bp, box, seed=0, min_dist=2.0, max_tries=200000):
                   strands, bases, n3s, n5s)
 
 
-def hairpin(stem_seq, loop_len, loop_base="T", rise=RISE, twist=TWIST, com_radius=COM_RADIUS):
+# backbone site = centre + c1 a1 + c2 a2 + c3 a3 with a2 = a3 x a1
+DNA1_BACK = (-0.4, 0.0, 0.0)        # oxDNA1
+DNA2_BACK = (-0.34, 0.3408, 0.0)    # oxDNA2 (grooved)
+RNA_BACK = (-0.4, 0.0, 0.2)         # oxRNA
+
+
+def _bridge(first, last, m, bulge, back=DNA2_BACK, step=0.75):
+    """m single-stranded nucleotides joining nucleotide ``first`` to ``last`` (each a
+    (centre, a1, a3) triple). Their backbone sites (centre + back . (a1, a3 x a1, a3))
+    lie on the straight segment between the anchors' backbone sites when that is at
+    least (m + 1) * step long, otherwise on a circular arc bulging along ``bulge``
+    whose consecutive backbone sites, anchors included, are exactly ``step`` apart.
+    a3 follows the chain, a1 is normal to the arc's plane (or across the segment), so
+    the base sites run parallel to the backbone sites and never crowd.
+    Returns centres, a1, a3 (m rows each)."""
+    c1, c2, c3 = back
+
+    def site(p, a1, a3):
+        p, a1, a3 = (np.asarray(v, float) for v in (p, a1, a3))
+        return p + c1 * a1 + c2 * np.cross(a3, a1) + c3 * a3
+
+    bs = site(*first)
+    be = site(*last)
+    if m == 0:
+        return np.zeros((0, 3)), np.zeros((0, 3)), np.zeros((0, 3))
+    chord = be - bs
+    c = np.linalg.norm(chord)
+    u = chord / c
+    w = np.asarray(bulge, float) - u * np.dot(bulge, u)
+    if np.linalg.norm(w) < 1e-9:
+        w = np.cross(u, [0.0, 0.0, 1.0]) if abs(u[2]) < 0.9 else np.cross(u, [1.0, 0.0, 0.0])
+    w /= np.linalg.norm(w)
+    B, A1, A3 = [], [], []
+    if c >= step * (m + 1):
+        for k in range(1, m + 1):
+            B.append(bs + chord * k / (m + 1))
+            A1.append(-w)
+            A3.append(u)
+    else:
+        # equal chords `step` on a circle: sin((m+1) x / 2) / sin(x / 2) = c / step for
+        # the angle x per step (the ratio falls from m + 1 to 0 on (0, 2 pi / (m + 1)))
+        lo, hi = 1e-9, 2 * math.pi / (m + 1) - 1e-9
+        for _ in range(200):
+            x = 0.5 * (lo + hi)
+            if math.sin((m + 1) * x / 2) / math.sin(x / 2) > c / step:
+                lo = x
+            else:
+                hi = x
+        x = 0.5 * (lo + hi)
+        r = step / (2 * math.sin(x / 2))
+        th = (m + 1) * x
+        h = r * math.cos(th / 2)
+        centre = 0.5 * (bs + be) - w * h
+        e1 = (bs - centre) / r
+        e2 = u - e1 * np.dot(u, e1)
+        e2 /= np.linalg.norm(e2)
+        # rotate from the first anchor through the bulge to the last: pick the sense
+        # that lands on it
+        if np.linalg.norm(centre + r * (math.cos(th) * e1 + math.sin(th) * e2) - be) > 1e-6 * max(1.0, r):
+            e2 = -e2
+        normal = np.cross(e1, e2)
+        for k in range(1, m + 1):
+            ang = x * k
+            b = centre + r * (math.cos(ang) * e1 + math.sin(ang) * e2)
+            tang = -math.sin(ang) * e1 + math.cos(ang) * e2
+            B.append(b); A1.append(normal); A3.append(tang)
+    B, A1, A3 = np.array(B), np.array(A1), np.array(A3)
+    return B - c1 * A1 - c2 * np.cross(A3, A1) - c3 * A3, A1, A3
+
+
+def hairpin(stem_seq, loop_len, loop_base="T", rise=RISE, twist=TWIST, com_radius=COM_RADIUS, back=DNA2_BACK):
     """One hairpin: stem ``stem_seq`` (5' half, listed 3'->5'), a poly-``loop_base`` loop of
     ``loop_len`` nt, then the complementary 3' half; one strand, centred on the origin.
 
-    The loop nucleotides sit on a circular arc above the stem that joins the two stem
-    ends, with a1 pointing to the arc's centre and a3 along the arc."""
+    The loop's backbone sites sit on a circular arc above the stem, spaced like the
+    FENE bond length (backbone site ``back``: oxDNA2 by default), with a3 along the
+    arc and a1 normal to its plane."""
     s = len(stem_seq)
     pos, a1, a3, _, _, _, _ = ideal_duplex(stem_seq, rise, twist, com_radius)
     up_p, up_a1, up_a3 = pos[:s], a1[:s], a3[:s]
     dn_p, dn_a1, dn_a3 = pos[s:], a1[s:], a3[s:]
-    start, end = up_p[-1], dn_p[0]
-    mid = 0.5 * (start + end)
-    chord = end - start
-    c = np.linalg.norm(chord)
-    u = chord / c
-    w = np.array([0.0, 0.0, 1.0])
-    w = w - u * np.dot(w, u)
-    w /= np.linalg.norm(w)
-    # arc of loop_len + 1 equal steps of length ~0.7 from start to end, bulging along +z
-    step = 0.7
-    n_seg = loop_len + 1
-    arc_len = step * n_seg
-    # solve for circle radius r with arc length arc_len over the chord c (theta in (0, 2pi))
-    lo, hi = 1e-6, 2 * math.pi - 1e-6
-    for _ in range(200):
-        th = 0.5 * (lo + hi)
-        if (c / (2 * math.sin(th / 2))) * th < arc_len:
-            lo = th
-        else:
-            hi = th
-    th = 0.5 * (lo + hi)
-    r = c / (2 * math.sin(th / 2))
-    h = r * math.cos(th / 2)  # centre offset from the chord midpoint (towards -w when th > pi)
-    centre = mid - w * h
-    loop_p, loop_a1, loop_a3 = [], [], []
-    v0 = start - centre
-    for k in range(1, loop_len + 1):
-        ang = th * k / n_seg
-        # rotate v0 about the axis n = u x w ... in the plane (u, w) from start to end
-        e1 = v0 / np.linalg.norm(v0)
-        e2 = u - e1 * np.dot(u, e1)
-        if np.linalg.norm(e2) < 1e-9:
-            e2 = w
-        e2 /= np.linalg.norm(e2)
-        p = centre + r * (math.cos(ang) * e1 + math.sin(ang) * e2)
-        radial = (centre - p) / r
-        tang = -math.sin(ang) * e1 + math.cos(ang) * e2
-        a1k = radial
-        a3k = tang - a1k * np.dot(tang, a1k)
-        a3k /= np.linalg.norm(a3k)
-        loop_p.append(p); loop_a1.append(a1k); loop_a3.append(a3k)
-    P = np.concatenate([up_p, np.array(loop_p).reshape(-1, 3), dn_p])
-    A1 = np.concatenate([up_a1, np.array(loop_a1).reshape(-1, 3), dn_a1])
-    A3 = np.concatenate([up_a3, np.array(loop_a3).reshape(-1, 3), dn_a3])
+    loop_p, loop_a1, loop_a3 = _bridge((up_p[-1], up_a1[-1], up_a3[-1]), (dn_p[0], dn_a1[0], dn_a3[0]), loop_len,
+                                       bulge=np.array([0.0, 0.0, 1.0]), back=back)
+    P = np.concatenate([up_p, loop_p.reshape(-1, 3), dn_p])
+    A1 = np.concatenate([up_a1, loop_a1.reshape(-1, 3), dn_a1])
+    A3 = np.concatenate([up_a3, loop_a3.reshape(-1, 3), dn_a3])
     bases = list(stem_seq) + [loop_base] * loop_len + [COMPLEMENT[b] for b in reversed(stem_seq)]
     n = len(bases)
     n3 = [i - 1 for i in range(n)]
@@ -251,3 +283,161 @@ def hairpin_ensemble(n_replicas, seed=3, stem=(4, 30), loop=(4, 10), box=30.0):
         out.append(System(p, q, [io.base_typeid(b) for b in bases], bonds, [box] * 3, strand,
                           bases, n3, n5))
     return out
+
+
+# ---------------------------------------------------------------- RNA cloverleaf
+# Idealised A-form helix for RNA (2.8 A rise = 0.3287 oxDNA length units, 11 bp per
+# turn, base-pair planes normal to the axis): backbone sites of consecutive
+# nucleotides then sit ~0.65 apart, inside the oxRNA FENE range 0.761 +- 0.25.
+RNA_RISE = 0.3287
+RNA_TWIST = 2.0 * math.pi / 11.0
+RNA_COMPLEMENT = {"A": "U", "U": "A", "G": "C", "C": "G"}
+
+
+def _rotation_z_to(u, phase):
+    """Rotation taking local +z to unit vector u, with a phase rotation about u."""
+    u = np.asarray(u, dtype=float)
+    u = u / np.linalg.norm(u)
+    ref = np.array([0.0, 0.0, 1.0]) if abs(u[2]) < 0.9 else np.array([1.0, 0.0, 0.0])
+    e1 = np.cross(ref, u)
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(u, e1)
+    c, s = math.cos(phase), math.sin(phase)
+    x = c * e1 + s * e2
+    y = -s * e1 + c * e2
+    return np.column_stack([x, y, u])
+
+
+def _stem_local(seq, rise, twist):
+    """Ideal stem along +z from z = 0 (first pair) upwards: strand 0 (seq) up, its
+    complement down. Returns pos, a1, a3 of strand 0 and of strand 1."""
+    pos, a1, a3, _, _, _, _ = ideal_duplex(seq.replace("U", "T"), rise, twist, COM_RADIUS)
+    n = len(seq)
+    shift = np.array([0.0, 0.0, 0.5 * (n - 1) * rise])
+    return pos[:n] + shift, a1[:n], a3[:n], pos[n:] + shift, a1[n:], a3[n:]
+
+
+def cloverleaf(seed=0, n_total=76, stems=(4, 7), loops=(3, 8), acceptor=7):
+    """One RNA strand of ``n_total`` nt with a seeded random sequence folded into an
+    idealised A-form cloverleaf: an acceptor stem closed by the two chain ends and
+    three hairpin arms (stems s ~ U{stems} bp, loops ~ U{loops} nt), joined by
+    single-stranded linkers that take the remaining nucleotides. The molecule lies in
+    the xy plane, arms at -x, +y, +x around the junction, the acceptor stem along -y;
+    stem partners are Watson-Crick complements. Returns (System, info) where info
+    lists the stems as (first, last, partner_first) index triples and the linker
+    lengths; the box is left to the caller (set ``system.box``)."""
+    rng = np.random.default_rng(seed)
+    s0 = int(acceptor)
+    for _ in range(10000):
+        arm_s = [int(rng.integers(stems[0], stems[1] + 1)) for _ in range(3)]
+        arm_l = [int(rng.integers(loops[0], loops[1] + 1)) for _ in range(3)]
+        rest = n_total - 2 * s0 - sum(2 * s + l for s, l in zip(arm_s, arm_l))
+        if 8 <= rest <= 16:
+            break
+    else:
+        raise RuntimeError("no cloverleaf layout fits n_total")
+    linkers = [rest // 4 + (1 if k < rest % 4 else 0) for k in range(4)]
+    alphabet = "ACGU"
+    seq_acc = "".join(rng.choice(list(alphabet), size=s0))
+    seq_arms = ["".join(rng.choice(list(alphabet), size=s)) for s in arm_s]
+
+    def build(R0, phases, step):
+        parts = []  # (pos, a1, a3, bases) in chain order
+        # acceptor strand 0 runs toward the junction (local +z -> +y), top at (0, -R0, 0)
+        p0, a10, a30, p1, a11, a31 = _stem_local(seq_acc, RNA_RISE, RNA_TWIST)
+        Ra = _rotation_z_to([0.0, 1.0, 0.0], phases[0])
+        top = np.array([0.0, 0.0, (s0 - 1) * RNA_RISE])
+        t = np.array([0.0, -R0, 0.0])
+        acc0 = ((p0 - top) @ Ra.T + t, a10 @ Ra.T, a30 @ Ra.T, list(seq_acc))
+        acc1 = ((p1 - top) @ Ra.T + t, a11 @ Ra.T, a31 @ Ra.T,
+                [RNA_COMPLEMENT[b] for b in reversed(seq_acc)])
+        arms = []
+        for k, (u, seq, ell) in enumerate(zip(([-1.0, 0, 0], [0, 1.0, 0], [1.0, 0, 0]), seq_arms, arm_l)):
+            q0, b10, b30, q1, b11, b31 = _stem_local(seq, RNA_RISE, RNA_TWIST)
+            Rk = _rotation_z_to(u, phases[k + 1])
+            tk = R0 * np.asarray(u)
+            up = (q0 @ Rk.T + tk, b10 @ Rk.T, b30 @ Rk.T)
+            dn = (q1 @ Rk.T + tk, b11 @ Rk.T, b31 @ Rk.T)
+            lp = _bridge((up[0][-1], up[1][-1], up[2][-1]), (dn[0][0], dn[1][0], dn[2][0]), ell,
+                         bulge=np.asarray(u), back=RNA_BACK, step=step)
+            arms.append((up, lp, dn, seq))
+        parts.append(acc0)
+        prev_end = (acc0[0][-1], acc0[1][-1], acc0[2][-1])
+        for k, (up, lp, dn, seq) in enumerate(arms):
+            lk = _bridge(prev_end, (up[0][0], up[1][0], up[2][0]), linkers[k], bulge=[0.0, 0.0, 1.0],
+                         back=RNA_BACK, step=step)
+            parts.append((lk[0], lk[1], lk[2], ["U"] * linkers[k]))
+            parts.append((up[0], up[1], up[2], list(seq)))
+            parts.append((lp[0], lp[1], lp[2], ["U"] * len(lp[0])))
+            parts.append((dn[0], dn[1], dn[2], [RNA_COMPLEMENT[b] for b in reversed(seq)]))
+            prev_end = (dn[0][-1], dn[1][-1], dn[2][-1])
+        lk = _bridge(prev_end, (acc1[0][0], acc1[1][0], acc1[2][0]), linkers[3], bulge=[0.0, 0.0, 1.0],
+                     back=RNA_BACK, step=step)
+        parts.append((lk[0], lk[1], lk[2], ["U"] * linkers[3]))
+        parts.append(acc1)
+        pos = np.concatenate([np.asarray(p[0]).reshape(-1, 3) for p in parts])
+        a1 = np.concatenate([np.asarray(p[1]).reshape(-1, 3) for p in parts])
+        a3 = np.concatenate([np.asarray(p[2]).reshape(-1, 3) for p in parts])
+        bases = [b for p in parts for b in p[3]]
+        return pos, a1, a3, bases
+
+    def quality(pos, a1, a3):
+        # worst deviation of a bonded backbone-site distance from the FENE r0 (oxRNA
+        # backbone site -0.4 a1 + 0.2 a3), and the closest approach of backbone sites
+        # and of base sites (+0.4 a1) of nucleotides that are not chain neighbours
+        back = pos - 0.4 * a1 + 0.2 * a3
+        base = pos + 0.4 * a1
+        fene = np.abs(np.linalg.norm(np.diff(back, axis=0), axis=1) - 0.761).max()
+        n = len(pos)
+        idx = np.arange(n)
+        far = np.abs(idx[:, None] - idx[None, :]) > 1
+        dbb = np.linalg.norm(back[:, None] - back[None], axis=2)[far].min()
+        dss = np.linalg.norm(base[:, None] - base[None], axis=2)[far].min()
+        return fene, dbb, dss
+
+    best = None
+    for step in (0.75, 0.7, 0.8, 0.65):
+        for R0 in np.arange(1.6, 4.01, 0.2):
+            phases = [0.0, 0.0, 0.0, 0.0]
+            for k in range(4):  # greedy phase choice: best worst backbone bond
+                cand = []
+                for ph in np.linspace(0, 2 * math.pi, 12, endpoint=False):
+                    trial = list(phases)
+                    trial[k] = ph
+                    cand.append((quality(*build(R0, trial, step)[:3])[0], ph))
+                phases[k] = min(cand)[1]
+            pos, a1, a3, bases = build(R0, phases, step)
+            fene, dbb, dss = quality(pos, a1, a3)
+            if fene < 0.17 and dbb > 0.6 and dss > 0.3:
+                best = (pos, a1, a3, bases)
+                break
+        if best is not None:
+            break
+    if best is None:
+        raise RuntimeError("could not lay out the cloverleaf without clashes")
+    pos, a1, a3, bases = best
+    n = len(bases)
+    assert n == n_total, (n, n_total)
+    quat = io.orientations_to_quaternions(a1, a3, check=False)
+    n3 = [i - 1 for i in range(n)]
+    n5 = [i + 1 if i < n - 1 else -1 for i in range(n)]
+    bonds = [(i, i + 1) for i in range(n - 1)]
+    typeid = [io.base_typeid("T" if b == "U" else b) for b in bases]
+    system = System(pos - pos.mean(axis=0), quat, typeid, bonds, [4.0 * 30.0] * 3, [0] * n, bases, n3, n5)
+    info = dict(arm_stems=arm_s, arm_loops=arm_l, acceptor=s0, linkers=linkers)
+    return system, info
+
+
+def cloverleaf_ensemble(n_replicas, seed=5, box=64.0, d_range=(2.0, 30.0), k_bias=1.0):
+    """``n_replicas`` copies of one seeded cloverleaf, each in its own cubic box of
+    side ``box``, with a harmonic end-to-end bias k (|r_last - r_first| - d_r)^2 whose
+    centres d_r are spaced uniformly over ``d_range``. Returns (systems, ends, k, d0)."""
+    base, _ = cloverleaf(seed=seed)
+    out = []
+    for _ in range(n_replicas):
+        out.append(System(base.pos.copy(), base.quat.copy(), base.typeid, base.bonds, [box] * 3, base.strand,
+                          base.bases, base.n3, base.n5))
+    ends = np.tile([0, base.n - 1], (n_replicas, 1))
+    d0 = np.linspace(d_range[0], d_range[1], n_replicas)
+    k = np.full(n_replicas, float(k_bias))
+    return out, ends, k, d0

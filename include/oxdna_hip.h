@@ -267,6 +267,32 @@ int oxdna_set_langevin(oxdna_ctx* ctx, double dt, double kT, double gamma, doubl
  * replays the step as a captured HIP graph. */
 int oxdna_run(oxdna_ctx* ctx, unsigned int nsteps, unsigned int term_mask, int use_graph);
 int oxdna_get_state(oxdna_ctx* ctx, double* pos, double* quat, double* vel, double* angmom);
+/* Batched replicas: the particles given to oxdna_set_state are n_replicas
+ * independent systems. Replica r holds particles [offsets[r], offsets[r+1]) and lives
+ * in its own periodic box boxes[3r..3r+2] (each length above twice the largest
+ * cut-off); no pair or bond crosses replicas (bonds that would are rejected). Each
+ * replica has at most 512 particles. oxdna_compute / oxdna_run / oxdna_get_forces /
+ * oxdna_get_state then act on all replicas at once, in one persistent launch per
+ * chunk of steps (one workgroup per replica, its state resident in LDS); replica r
+ * draws its Langevin noise with seed + r, i.e. exactly the stream of a one-system
+ * context holding replica r alone with that seed. Term energies are summed over
+ * the replicas; the virial is not computed (zero). n_replicas = 0 returns to one
+ * system. A later oxdna_set_state with another particle count clears the replicas.
+ * No counterpart in the reference (one system per HOOMD simulation there). */
+int oxdna_set_replicas(oxdna_ctx* ctx, unsigned int n_replicas, const unsigned int* offsets,
+                       const double* boxes);
+/* Per-replica harmonic bias k[r] (|r_e - r_s| - d0[r])^2 on the centres of particles
+ * s = ends[2r] and e = ends[2r+1] (indices local to replica r); k[r] = 0 leaves a
+ * replica unbiased, NULL arrays remove the bias. Energy slot OXDNA_E_BIAS of
+ * oxdna_get_replica_energies. Not a term of the reference: a new force for
+ * umbrella-style ensembles. */
+int oxdna_set_replica_bias(oxdna_ctx* ctx, const int32_t* ends, const double* k, const double* d0);
+#define OXDNA_E_BIAS 8
+#define OXDNA_N_REPLICA_ENERGY 9
+/* Per-replica energies of the last compute: n_replicas x 9 doubles (the 8
+ * split_energy slots, then the bias). */
+int oxdna_get_replica_energies(oxdna_ctx* ctx, double* out);
+
 /* Blocks until the context's stream is idle. */
 int oxdna_synchronize(oxdna_ctx* ctx);
 /* Measurement: times nrep launches of the fused force kernel for term_mask at the

@@ -1,0 +1,158 @@
+"""GPU parity of the batched replica path (oxdna_set_replicas, one persistent launch
+for all replicas) against the FP64 CPU oracle, replica by replica, and against
+one-system contexts run with the same random streams.
+
+Tolerances are the one-system ones (test_gpu_parity): per-term energy
+|dE| <= 1e-4 |E_ref| + 1e-5 N, per-particle force / torque
+|d| <= 2e-4 max|ref| + 2e-4. The bias is checked against its closed form
+k (|r_e - r_s| - d0)^2 and its analytic force (FP64 on the host).
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle as O
+from oxdna import ensemble, sim, systems
+from oxdna.model import dna1, dna2, rna2
+
+pytestmark = pytest.mark.gpu
+
+E_REL, E_PER_NT = 1e-4, 1e-5
+F_REL, F_ABS = 2e-4, 2e-4
+KT = dna1.kT_from_temperature(296.15)
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _ensemble(reps, forces, seed=3, langevin=False):
+    ens = ensemble.ReplicaEnsemble(device=sim.GPU(), seed=seed)
+    ens.create_state_from_snapshots([r.snapshot() for r in reps])
+    methods = [sim.Langevin(kT=KT)] if langevin else []
+    ens.operations.integrator = sim.Integrator(dt=0.003, forces=forces, methods=methods)
+    return ens
+
+
+def _check_replicas(reps, forces, label, bias=None):
+    ens = _ensemble(reps, forces)
+    if bias is not None:
+        ens.bias = bias
+    ens.run(0)
+    got_e = ens.replica_energies()
+    _, f4, t3, _ = ens._eval(ens._mask)
+    off = ens.offsets
+    problems = []
+    worst = [0.0, 0.0, 0.0]
+    for r, rep in enumerate(reps):
+        ref = O.from_system(rep, forces).compute()
+        F_ref, T_ref = ref["F"].copy(), ref["T"].copy()
+        e_ref = np.zeros(9)
+        e_ref[:8] = ref["E"]
+        if bias is not None:
+            s, e = bias.ends[r]
+            d = rep.pos[e] - rep.pos[s]
+            d -= rep.box * np.rint(d / rep.box)
+            dist = np.linalg.norm(d)
+            e_ref[8] = bias.k[r] * (dist - bias.d0[r]) ** 2
+            fe = -2.0 * bias.k[r] * (dist - bias.d0[r]) * d / dist
+            F_ref[e] += fe
+            F_ref[s] -= fe
+        n = rep.n
+        dE = np.abs(got_e[r] - e_ref)
+        tolE = E_REL * np.abs(e_ref) + E_PER_NT * n
+        sl = slice(int(off[r]), int(off[r + 1]))
+        dF = np.abs(f4[sl, :3] - F_ref).max()
+        dT = np.abs(t3[sl] - T_ref).max()
+        fmax, tmax = np.abs(F_ref).max(), np.abs(T_ref).max()
+        worst = [max(worst[0], (dE / np.maximum(np.abs(e_ref), 1e-300)).max(where=e_ref != 0, initial=0.0)),
+                 max(worst[1], dF / fmax), max(worst[2], dT / tmax)]
+        if not np.all(dE <= tolE):
+            problems.append(f"replica {r} (n={n}): energies {got_e[r]} vs {e_ref}")
+        if not dF <= F_REL * fmax + F_ABS:
+            problems.append(f"replica {r}: force error {dF} (max {fmax})")
+        if not dT <= F_REL * tmax + F_ABS:
+            problems.append(f"replica {r}: torque error {dT} (max {tmax})")
+    _say(f"[{label}] {len(reps)} replicas: worst rel dE={worst[0]:.1e}, dF/max={worst[1]:.1e}, "
+         f"dT/max={worst[2]:.1e}")
+    assert not problems, f"{label}: " + "; ".join(problems[:5])
+    return ens
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_hairpin_replicas_match_oracle(fused):
+    reps = [H.jiggle(h, seed=10 + k, dx=0.03, dq=0.05) for k, h in enumerate(systems.hairpin_ensemble(12, seed=3))]
+    forces, _ = dna2.forces(fused=fused)
+    _check_replicas(reps, forces, f"hairpins dna2 fused={fused}")
+
+
+def test_oxdna1_duplex_replicas_match_oracle():
+    reps = [H.jiggle(H.cfg1_system(k), seed=20 + k) for k in range(5)]
+    forces, _ = dna1.forces(fused=True)
+    _check_replicas(reps, forces, "duplexes dna1")
+
+
+def test_rna_cloverleaf_replicas_with_bias_match_oracle():
+    reps, ends, k, d0 = systems.cloverleaf_ensemble(6, seed=5, d_range=(2.0, 30.0), k_bias=0.5)
+    reps = [H.jiggle(r, seed=40 + i, dx=0.02, dq=0.03) for i, r in enumerate(reps)]
+    forces, _ = rna2.forces(fused=True)
+    bias = ensemble.HarmonicBias(ends, k, d0)
+    ens = _check_replicas(reps, forces, "cloverleaf rna2 + bias", bias=bias)
+    # bias energies are large for the far centres and vanish nowhere
+    e = ens.replica_energies()[:, 8]
+    assert np.all(e > 0) and e[-1] > e[0]
+
+
+def test_uneven_sizes_one_launch_up_to_512():
+    """Replicas from 1 to 512 nt in one launch (workgroup of 512 threads)."""
+    rng = np.random.default_rng(2)
+    big = systems.duplex_gas(12, 20, box=40.0, seed=4, min_dist=1.5)  # 480 nt
+    single = systems.duplex_system(systems.random_sequence(rng, 4), box=20.0)
+    mid = H.jiggle(H.cfg1_system(6), seed=7)
+    reps = [single, big, mid]
+    forces, _ = dna2.forces(fused=True)
+    _check_replicas(reps, forces, "uneven sizes")
+
+
+def test_replicas_follow_one_system_runs():
+    """Replica r of an ensemble with seed s draws the random stream of a one-system run
+    with seed s + r: after 50 Langevin steps both paths agree up to FP32 summation
+    order (the pair sums run in different orders)."""
+    base = systems.hairpin_ensemble(3, seed=8)
+    forces, _ = dna2.forces(fused=True)
+    ens = _ensemble(base, forces, seed=100, langevin=True)
+    ens.run(50)
+    snap = ens.state.get_snapshot()
+    off = ens.offsets
+    for r, rep in enumerate(base):
+        f2, _ = dna2.forces(fused=True)
+        s = sim.Simulation(device=sim.GPU(), seed=100 + r)
+        s.create_state_from_snapshot(rep.snapshot())
+        s.operations.integrator = sim.Integrator(dt=0.003, forces=f2, methods=[sim.Langevin(kT=KT)])
+        s.run(50)
+        one = s.state.get_snapshot()
+        sl = slice(int(off[r]), int(off[r + 1]))
+        dx = np.abs(snap.particles.position[sl] - one.particles.position).max()
+        dv = np.abs(snap.particles.velocity[sl] - one.particles.velocity).max()
+        moved = np.abs(one.particles.position - rep.pos).max()
+        _say(f"replica {r} (n={rep.n}): |dx|={dx:.2e} |dv|={dv:.2e} (moved {moved:.3f})")
+        assert dx < 1e-3 and dv < 1e-2
+        assert moved > 1e-3
+
+
+def test_replica_run_is_stable_and_reports_energies():
+    reps = systems.hairpin_ensemble(32, seed=9)
+    forces, _ = dna2.forces(fused=True)
+    ens = _ensemble(reps, forces, seed=1, langevin=True)
+    ens.run(3000)
+    e = ens.replica_energies()
+    pos = ens.state.get_snapshot().particles.position
+    hb = e[:, 4] / np.diff(ens.offsets)
+    _say(f"after 3000 steps: mean HB/nt {hb.mean():.4f}, min {hb.min():.4f}, fene max {e[:, 0].max():.3f}")
+    assert np.isfinite(pos).all() and np.isfinite(e).all()
+    assert np.all(e[:, 0] < 1e6)          # no broken backbone bond
+    assert hb.mean() < -0.1               # stems stay paired on average
