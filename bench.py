@@ -28,12 +28,16 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32 (published)
 TERM_NAMES = ["excvol", "hbond", "cross", "coax", "debye", "hbondcross", "bonded"]
 
 
-def flops_per_launch(counts, census):
-    """Algorithmic flops of one fused force launch: directed evaluation counts per term
-    (inside cut-off, nonzero energy) x per-evaluation flops of the op census."""
+def flops_per_launch(counts, census, evals_per_pair=1):
+    """Algorithmic flops of one force evaluation: evaluation counts per term (inside
+    cut-off, nonzero energy) x flops of the op census, which prices one evaluation
+    of a pair (forces on both, both torques) or a bond. Counts from the one-system
+    kernel are directed (every pair and bond seen from both ends: evals_per_pair=2)
+    and are halved, so the figure is the work any evaluation must do, not the
+    full-list kernel's duplicated work."""
     total = 0.0
     for t, name in enumerate(TERM_NAMES):
-        n_in, n_nz = counts[2 * t], counts[2 * t + 1]
+        n_in, n_nz = counts[2 * t] / evals_per_pair, counts[2 * t + 1] / evals_per_pair
         if n_in == 0:
             continue
         c = census[name]
@@ -68,6 +72,80 @@ def cpu_baseline(system, forces, seconds=12.0):
                        f"each time, no integration) of the same {system.n}-nt system, 1 thread")
 
 
+def _timed_run(s, steps):
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.run(steps)
+    s.synchronize()
+    return time.perf_counter() - t0
+
+
+def _langevin(kT):
+    from oxdna import sim
+    return [sim.Langevin(kT=kT)]
+
+
+def bench_batched(name, reps, forces, kT, steps, warmup, census, bias=None, sample=32, sample_steps=500):
+    """One batched ensemble (all replicas, one persistent launch per chunk of steps)
+    against the plain one-system path run replica by replica (a sample of replicas
+    spread over the ensemble, each in its own context with its graph-captured step)."""
+    import ctypes as C
+    import numpy as np
+    from oxdna import ensemble, sim
+    n_rep = len(reps)
+    ens = ensemble.ReplicaEnsemble(device=sim.GPU(0), seed=11)
+    ens.create_state_from_snapshots([r.snapshot() for r in reps])
+    ens.operations.integrator = sim.Integrator(dt=0.003, forces=forces, methods=_langevin(kT))
+    if bias is not None:
+        ens.bias = bias
+    ens.run(max(warmup, 1))
+    dt = _timed_run(ens, steps)
+    rate = n_rep * steps / dt
+    e = ens.replica_energies()
+    assert np.isfinite(e).all(), "non-finite replica energies"
+    lib, ctx = ens._lib, ens._ctx
+    avg_ms = C.c_double()
+    assert lib.oxdna_time_forces(ctx, ens._mask, 200, C.byref(avg_ms)) == 0
+    counts = (C.c_uint64 * 14)()
+    assert lib.oxdna_count_evaluations(ctx, ens._mask, counts) == 0
+    flops = flops_per_launch(list(counts), census)
+    achieved = flops / (avg_ms.value * 1e-3) / 1e12
+    tpp, nt = C.c_int(), C.c_int()
+    assert lib.oxdna_get_tuning(ctx, C.byref(tpp), C.byref(nt)) == 0
+    # plain path, one replica at a time
+    pick = np.linspace(0, n_rep - 1, min(sample, n_rep)).round().astype(int) if sample > 0 else []
+    plain_steps, plain_time = 0, 0.0
+    for r in pick:
+        s = sim.Simulation(device=sim.GPU(0), seed=11 + int(r))
+        s.create_state_from_snapshot(reps[r].snapshot())
+        s.operations.integrator = sim.Integrator(dt=0.003, forces=forces, methods=_langevin(kT))
+        s.run(64)
+        plain_time += _timed_run(s, sample_steps)
+        plain_steps += sample_steps
+        del s
+    plain_rate = plain_steps / plain_time if plain_steps else None
+    sizes = np.diff(ens.offsets)
+    return {
+        "workload": name,
+        "replicas": n_rep,
+        "nucleotides": int(sizes.sum()),
+        "replica_size_min_max": [int(sizes.min()), int(sizes.max())],
+        "steps": steps,
+        "replica_steps_per_s": round(rate, 1),
+        "ensemble_steps_per_s": round(steps / dt, 1),
+        "plain_per_replica_replica_steps_per_s": round(plain_rate, 1) if plain_rate else None,
+        "plain_sample": f"{len(pick)} replicas spread over the ensemble, {sample_steps} steps each, "
+                        "one context per replica (graph-captured step), run one after another",
+        "speedup_vs_plain": round(rate / plain_rate, 2) if plain_rate else None,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 6), "traffic": None,
+                     "kernel": f"k_ensemble<{nt.value}> (forces only)", "kernel_ms": round(avg_ms.value, 5),
+                     "flops_per_launch": flops},
+        "energy_per_nt": {k: round(float(v) / int(sizes.sum()), 6)
+                          for k, v in zip(ensemble.REPLICA_ENERGY_NAMES, e.sum(axis=0))},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +153,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched replica workloads")
+    ap.add_argument("--batched-steps", type=int, default=2000)
+    ap.add_argument("--plain-sample", type=int, default=32,
+                    help="replicas run one by one on the plain path for the comparison (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the replica / "
@@ -119,9 +201,11 @@ def main():
     counts = list(counts)
     with open(os.path.join(ROOT, "hoomd-oxdna_amd", "oxdna", "data", "census_flops.json")) as fh:
         census = json.load(fh)["oxdna2_fused"]
-    flops = flops_per_launch(counts, census)
+    flops = flops_per_launch(counts, census, evals_per_pair=2)
     achieved = flops / (avg_ms.value * 1e-3) / 1e12
-    pair_evals = sum(counts[2 * t] for t in range(6))
+    pair_evals = sum(counts[2 * t] for t in range(6)) // 2
+    tpp = C.c_int()
+    assert lib.oxdna_get_tuning(ctx, C.byref(tpp), None) == 0
     stats = s.stats()
 
     line = {
@@ -142,12 +226,29 @@ def main():
                    "n_nucleotides": system.n, "graph": not args.no_graph},
         "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 6),
-                     "traffic": traffic_from_profile(), "kernel": "k_forces",
+                     "traffic": traffic_from_profile(), "kernel": f"k_forces<{tpp.value}>",
                      "kernel_ms": round(avg_ms.value, 5), "flops_per_launch": flops},
         "pair_evaluations_per_s": round(pair_evals * steps_per_s, 1),
         "nlist_builds": stats["nlist_builds"],
         "energy_per_nt": {k: round(v / system.n, 6) for k, v in e.items()},
     }
+    if not args.no_batched:
+        from oxdna import ensemble
+        from oxdna.model import rna2
+        kT = dna1.kT_from_temperature(296.15)
+        hp = systems.hairpin_ensemble(512, seed=3, box=30.0)
+        fh, _ = dna2.forces(fused=True)
+        b1 = bench_batched("512 oxDNA2 hairpins (stem 4-30 bp, poly-T loop 4-10 nt), box 30 each, "
+                           "Langevin dt=0.003, fused HB+CS, 0.5 M, 296.15 K", hp, fh, kT,
+                           args.batched_steps, 200, census, sample=args.plain_sample)
+        cl, ends, kb, d0 = systems.cloverleaf_ensemble(256, seed=5, box=64.0, d_range=(2.0, 30.0), k_bias=0.02)
+        fr, _ = rna2.forces(fused=True)
+        with open(os.path.join(ROOT, "hoomd-oxdna_amd", "oxdna", "data", "census_flops.json")) as fh_:
+            census_rna = json.load(fh_).get("oxrna2_fused", census)
+        b2 = bench_batched("256 oxRNA2 76-nt cloverleaves, harmonic end-to-end bias k=0.02, centres 2..30, "
+                           "box 64 each, Langevin dt=0.003, 1 M, 296.15 K", cl, fr, kT, args.batched_steps, 200,
+                           census_rna, bias=ensemble.HarmonicBias(ends, kb, d0), sample=args.plain_sample)
+        line["batched"] = [b1, b2]
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(system, forces)
     print(json.dumps(line), flush=True)

@@ -49,10 +49,13 @@ __global__ void k_frames(uint32_t N, const double4* __restrict__ quat, const dou
 }
 
 __global__ void k_cell_assign(CellArgs a) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.N) return;
-    double4 p = a.pos[i];
-    bin_particle(a, i, p.x, p.y, p.z);
+    uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    uint32_t occ = 0;
+    if (i < a.N) {
+        double4 p = a.pos[i];
+        occ = bin_particle(a, i, p.x, p.y, p.z);
+    }
+    wave_atomic_max(&a.flags->max_cell, occ);
 }
 
 __global__ void k_zero_u32(uint32_t* p, size_t n) {
@@ -114,7 +117,7 @@ __device__ __forceinline__ uint32_t scan_cells(const NbrArgs& nb, const double4&
 template<int TPP, bool WANT_E, bool WANT_V, bool COUNT>
 __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
     extern __shared__ uint32_t lds_rows[];
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t tid = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t i = tid / TPP;
     const uint32_t lane = tid % TPP;
     const bool active = i < a.N;
@@ -326,12 +329,8 @@ __global__ void k_reduce_energy(uint32_t nblocks, const double* __restrict__ par
 }
 
 // ------------------------------------------------------------------ integrator
-// First half of the step (lv_first_half), then the new frames, cell binning, and the
-// skin test that requests a neighbour-list rebuild.
-__global__ void k_step1(StepArgs a) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) a.flags->step += 1ull;
-    if (i >= a.N) return;
+// One particle of step1; returns its cell occupancy (0 without a cell list).
+__device__ __forceinline__ uint32_t step1_particle(const StepArgs& a, uint32_t i) {
     const Langevin lv{a.dt, a.kT, a.gamma, a.gamma_r, a.mass, a.inertia};
     double4 x = a.pos[i], v = a.vel[i], q = a.quat[i], p = a.angmom[i];
     const float4 f = a.force[i], t = a.torque[i];
@@ -344,19 +343,31 @@ __global__ void k_step1(StepArgs a) {
     f2.w = 0.f;
     a.fr0[i] = f0; a.fr1[i] = f1; a.fr2[i] = f2;
     if (a.cells.count) {
-        bin_particle(a.cells, i, x.x, x.y, x.z);
+        const uint32_t occ = bin_particle(a.cells, i, x.x, x.y, x.z);
         double4 l = a.pos_last[i];
         double dx = min_image(x.x - l.x, a.L[0], a.Linv[0]);
         double dy = min_image(x.y - l.y, a.L[1], a.Linv[1]);
         double dz = min_image(x.z - l.z, a.L[2], a.Linv[2]);
         if (dx * dx + dy * dy + dz * dz > a.skin_half2) atomicOr(&a.flags->rebuild, 1u);
+        return occ;
     }
+    return 0;
+}
+
+// First half of the step (lv_first_half), then the new frames, cell binning, and the
+// skin test that requests a neighbour-list rebuild.
+__global__ void k_step1(StepArgs a) {
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i == 0) a.flags->step += 1ull;
+    uint32_t occ = 0;
+    if (i < a.N) occ = step1_particle(a, i);
+    if (a.cells.count) wave_atomic_max(&a.flags->max_cell, occ);
 }
 
 // Second half (lv_second_half); also clears this particle's cell count for the next
 // step's binning.
 __global__ void k_step2(StepArgs a) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i == 0) a.flags->rebuild = 0u;
     if (i >= a.N) return;
     const Langevin lv{a.dt, a.kT, a.gamma, a.gamma_r, a.mass, a.inertia};
@@ -938,13 +949,15 @@ void autotune(oxdna_ctx* c, uint32_t m) {
         if (lds_bytes(c, tpp) > kMaxLds) continue;
         c->tpp = tpp;
         for (int w = 0; w < 3; ++w) launch_forces(c, m, 0, false, false, 0);
-        HIP_OK(hipEventRecord(e0, c->stream));
-        for (int r = 0; r < 20; ++r) launch_forces(c, m, 0, false, false, 0);
-        HIP_OK(hipEventRecord(e1, c->stream));
-        HIP_OK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) { best = ms; best_tpp = tpp; }
+        for (int trial = 0; trial < 3; ++trial) {  // best of three: timing noise picks no loser
+            HIP_OK(hipEventRecord(e0, c->stream));
+            for (int r = 0; r < 20; ++r) launch_forces(c, m, 0, false, false, 0);
+            HIP_OK(hipEventRecord(e1, c->stream));
+            HIP_OK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; best_tpp = tpp; }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1078,35 +1091,64 @@ EnsArgs ens_args(oxdna_ctx* c, uint32_t mask, uint32_t nsteps, uint32_t nevals, 
     a.nevals = nevals;
     a.want_e = want_e;
     a.counts = nullptr;
+    // near: inside any cut-off but Debye-Hueckel's; far: inside Debye-Hueckel's only
+    float rn = 0.f, rf = 0.f;
+    for (int t = 0; t < 6; ++t) {
+        if (!((mask >> t) & 1u)) continue;
+        for (int k = 0; k < 16; ++k) {
+            float r2 = c->P.rcutsq[t][k];
+            if (t == T_DEBYE) rf = std::max(rf, r2);
+            else rn = std::max(rn, r2);
+        }
+    }
+    a.rnear2 = rn;
+    a.rfar2 = std::max(rf, rn);
     return a;
 }
 
+// Smallest replica capacity (power of two, 64..512) that holds the largest replica.
+int ens_nmax(const oxdna_ctx* c) {
+    int nmax = 64;
+    while ((uint32_t)nmax < c->rep_nmax) nmax *= 2;
+    return nmax;
+}
+
+template<int NMAX>
+void launch_ens_nt(int nt, dim3 g, hipStream_t st, const EnsArgs& a) {
+    switch (nt) {
+    case 64: if constexpr (NMAX <= 64) { hipLaunchKernelGGL((k_ensemble<NMAX, 64, false>), g, dim3(64), 0, st, a); return; } break;
+    case 128: if constexpr (NMAX <= 128) { hipLaunchKernelGGL((k_ensemble<NMAX, 128, false>), g, dim3(128), 0, st, a); return; } break;
+    case 256: if constexpr (NMAX <= 256) { hipLaunchKernelGGL((k_ensemble<NMAX, 256, false>), g, dim3(256), 0, st, a); return; } break;
+    case 512: hipLaunchKernelGGL((k_ensemble<NMAX, 512, false>), g, dim3(512), 0, st, a); return;
+    default: break;
+    }
+    throw Fail("internal: unsupported replica workgroup size");
+}
+
+// Launches the replica kernel with workgroup size nt (count: the counting variant).
 void launch_ens(oxdna_ctx* c, int nt, bool count, const EnsArgs& a) {
-    if (nt < 64 || (uint32_t)nt < c->rep_nmax || a.n_rep == 0 || !a.offsets || !a.order || !a.P)
+    const int nmax = ens_nmax(c);
+    if (nt < nmax || a.n_rep == 0 || !a.offsets || !a.order || !a.P || c->rep_nmax > (uint32_t)nmax)
         throw Fail("internal: replica kernel launch does not fit the replicas");
     dim3 g(a.n_rep);
     if (count) {
-        hipLaunchKernelGGL((k_ensemble<512, true>), g, dim3(512), 0, c->stream, a);
+        hipLaunchKernelGGL((k_ensemble<512, 512, true>), g, dim3(512), 0, c->stream, a);
         return;
     }
-    switch (nt) {
-    case 64: hipLaunchKernelGGL((k_ensemble<64, false>), g, dim3(64), 0, c->stream, a); break;
-    case 128: hipLaunchKernelGGL((k_ensemble<128, false>), g, dim3(128), 0, c->stream, a); break;
-    case 256: hipLaunchKernelGGL((k_ensemble<256, false>), g, dim3(256), 0, c->stream, a); break;
-    case 512: hipLaunchKernelGGL((k_ensemble<512, false>), g, dim3(512), 0, c->stream, a); break;
-    default: throw Fail("internal: unsupported replica workgroup size");
+    switch (nmax) {
+    case 64: launch_ens_nt<64>(nt, g, c->stream, a); break;
+    case 128: launch_ens_nt<128>(nt, g, c->stream, a); break;
+    case 256: launch_ens_nt<256>(nt, g, c->stream, a); break;
+    case 512: launch_ens_nt<512>(nt, g, c->stream, a); break;
+    default: throw Fail("internal: unsupported replica capacity");
     }
 }
 
-int ens_default_nt(const oxdna_ctx* c) {
-    int nt = 64;
-    while ((uint32_t)nt < c->rep_nmax) nt *= 2;
-    return nt;
-}
+int ens_default_nt(const oxdna_ctx* c) { return std::max(256, ens_nmax(c)); }
 
-// Autotuner: workgroup size of the replica kernel (the smallest that holds the
-// largest replica, and each doubling up to 512), timed as forces-only launches
-// (the state is not changed) with HIP events; keeps the fastest.
+// Autotuner: workgroup size of the replica kernel (from the replica capacity up to
+// 512 threads, every power of two), timed as forces-only launches (the state is not
+// changed) with HIP events; keeps the fastest.
 void autotune_ens(oxdna_ctx* c, uint32_t m) {
     if (c->ens_tuned_mask == m && c->ens_nt != 0) return;
     float best = 1e30f;
@@ -1114,16 +1156,18 @@ void autotune_ens(oxdna_ctx* c, uint32_t m) {
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
-    for (int nt = ens_default_nt(c); nt <= (int)kEnsMaxN; nt *= 2) {
+    for (int nt = ens_nmax(c); nt <= 512; nt *= 2) {
         EnsArgs a = ens_args(c, m, 0, 8, 0);
         launch_ens(c, nt, false, a);
-        HIP_OK(hipEventRecord(e0, c->stream));
-        for (int rep = 0; rep < 3; ++rep) launch_ens(c, nt, false, a);
-        HIP_OK(hipEventRecord(e1, c->stream));
-        HIP_OK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) { best = ms; best_nt = nt; }
+        for (int trial = 0; trial < 3; ++trial) {
+            HIP_OK(hipEventRecord(e0, c->stream));
+            launch_ens(c, nt, false, a);
+            HIP_OK(hipEventRecord(e1, c->stream));
+            HIP_OK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; best_nt = nt; }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1685,6 +1729,14 @@ int oxdna_get_stats(oxdna_ctx* c, uint64_t* n_builds, uint32_t* max_neigh, uint6
         if (n_builds) *n_builds = f.builds;
         if (max_neigh) *max_neigh = c->max_neigh;
         if (step) *step = c->n_rep ? c->host_step : f.step;
+    });
+}
+
+int oxdna_get_tuning(oxdna_ctx* c, int* lanes_per_particle, int* replica_workgroup) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (lanes_per_particle) *lanes_per_particle = c->tpp;
+        if (replica_workgroup) *replica_workgroup = c->ens_nt;
     });
 }
 

@@ -130,12 +130,31 @@ struct CellArgs {
     DevFlags* flags;
 };
 
-__device__ __forceinline__ void bin_particle(const CellArgs& a, uint32_t i, double x, double y, double z) {
+// Bins particle i; returns the occupancy it saw (slot + 1) for the caller's
+// wave-reduced update of flags->max_cell.
+__device__ __forceinline__ uint32_t bin_particle(const CellArgs& a, uint32_t i, double x, double y, double z) {
     uint32_t c = cell_index(x, y, z, a.Linv, a.nc);
     uint32_t slot = atomicAdd(&a.count[c], 1u);
     if (slot < a.cap) a.members[(size_t)c * a.cap + slot] = i;
     else atomicOr(&a.flags->overflow, 1u);
-    atomicMax(&a.flags->max_cell, slot + 1);
+    return slot + 1;
+}
+
+// atomicMax of v over a whole wavefront: one atomic per wave instead of one per lane
+// (every lane of the wave must call it; v = 0 contributes nothing).
+__device__ __forceinline__ void wave_atomic_max(uint32_t* p, uint32_t v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, m));
+    if ((threadIdx.x & 63u) == 0u && v) atomicMax(p, v);
+}
+
+// Workgroups go to the 8 XCDs round-robin (workgroup b runs on XCD b mod 8, each XCD
+// with its own L2). This renumbering gives XCD x one contiguous range of logical
+// workgroups, so particles with close indices (one duplex, one strand) share an L2.
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t G = gridDim.x, b = blockIdx.x;
+    const uint32_t per = G / 8u, rem = G % 8u, x = b % 8u;
+    return x * per + min(x, rem) + b / 8u;
 }
 
 // In-kernel neighbour-row construction (scan_cells in engine.hip).
