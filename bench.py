@@ -48,7 +48,7 @@ def flops_per_launch(counts, census, evals_per_pair=1):
 
 def traffic_from_profile():
     """HBM bytes per force launch from the committed rocprofv3 counter summary, if any."""
-    path = os.path.join(ROOT, "profiles", "forces_8k_pmc.json")
+    path = os.path.join(ROOT, "profiles", "k_forces_oxdna2_8k_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -146,6 +146,26 @@ def bench_batched(name, reps, forces, kT, steps, warmup, census, bias=None, samp
     }
 
 
+def copy_probe(mib=256, reps=5):
+    """Device-to-device copies of a known size: the reference reading for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of a profiled run (bench.py --copy-probe)."""
+    import ctypes as C
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    n = C.c_size_t(mib << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), n) == 0 and hip.hipMalloc(C.byref(b), n) == 0
+    assert hip.hipMemset(a, 1, n) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, n, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": mib << 20, "copies": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,7 +177,12 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=2000)
     ap.add_argument("--plain-sample", type=int, default=32,
                     help="replicas run one by one on the plain path for the comparison (0: skip)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run 256 MiB device copies (counter calibration for profiles)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the replica / "
               "domain sharding across GPUs is described in DESIGN.md and not launched here.",
