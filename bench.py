@@ -8,11 +8,20 @@ centres in a cubic box of side 200 (8,000 nt, 1e-3 nt per unit^3), oxDNA2
 sequence-averaged, HB + cross-stacking fused, Debye-Hueckel at 0.5 M, T = 296.15 K,
 dt = 0.003, gamma = gamma_r = 1. State stays resident in HBM during the timed region.
 
+Then, in the same run ("batched" in the JSON line), two batched replica ensembles in one
+persistent launch per 2048 steps, each beside the plain one-system path run replica by
+replica: 512 oxDNA2 hairpins (stem 4-30 bp, poly-T loop 4-10 nt, box 30 each) and 256
+oxRNA2 76-nt cloverleaves with a harmonic end-to-end bias (centres 2..30, box 64 each).
+
 Prints one JSON line (see the fields below). Options:
-    --gpus N     only 1 is supported (one GPU)
-    --steps K    timed steps (default 10000)
-    --warmup W   untimed steps before timing (default 1000)
-    --no-cpu     skip the CPU baseline leg
+    --gpus N           only 1 is supported (one GPU)
+    --steps K          timed steps (default 10000)
+    --warmup W         untimed steps before timing (default 1000)
+    --no-cpu           skip the CPU baseline leg
+    --no-batched       skip the batched ensembles
+    --batched-steps S  timed steps of each ensemble (default 2000)
+    --plain-sample R   replicas run one by one on the plain path (default 32; 0 skips)
+    --copy-probe       only 256 MiB device copies (counter calibration for profiles/)
 """
 
 import argparse
@@ -112,6 +121,9 @@ def bench_batched(name, reps, forces, kT, steps, warmup, census, bias=None, samp
     achieved = flops / (avg_ms.value * 1e-3) / 1e12
     tpp, nt = C.c_int(), C.c_int()
     assert lib.oxdna_get_tuning(ctx, C.byref(tpp), C.byref(nt)) == 0
+    nmax = 64
+    while nmax < int(np.diff(ens.offsets).max()):
+        nmax *= 2
     # plain path, one replica at a time
     pick = np.linspace(0, n_rep - 1, min(sample, n_rep)).round().astype(int) if sample > 0 else []
     plain_steps, plain_time = 0, 0.0
@@ -139,7 +151,7 @@ def bench_batched(name, reps, forces, kT, steps, warmup, census, bias=None, samp
         "speedup_vs_plain": round(rate / plain_rate, 2) if plain_rate else None,
         "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 6), "traffic": None,
-                     "kernel": f"k_ensemble<{nt.value}> (forces only)", "kernel_ms": round(avg_ms.value, 5),
+                     "kernel": f"k_ensemble<{nmax}, {nt.value}> (forces only)", "kernel_ms": round(avg_ms.value, 5),
                      "flops_per_launch": flops},
         "energy_per_nt": {k: round(float(v) / int(sizes.sum()), 6)
                           for k, v in zip(ensemble.REPLICA_ENERGY_NAMES, e.sum(axis=0))},
