@@ -178,6 +178,48 @@ def copy_probe(mib=256, reps=5):
     print(json.dumps({"copy_probe_bytes": mib << 20, "copies": reps}), flush=True)
 
 
+def bench_large(model_name, steps, warmup, census_all):
+    """4096 ideal 64-bp duplexes (524,288 nt) in a box of 640 (2e-3 nt per unit^3),
+    cell list with a 10 % skin, fused Langevin steps."""
+    import ctypes as C
+    from oxdna import sim, systems
+    from oxdna.model import dna1, dna2
+    model = {"dna1": dna1, "dna2": dna2}[model_name]
+    t0 = time.perf_counter()
+    system = systems.duplex_gas(4096, 64, box=640.0, seed=3)
+    t_gen = time.perf_counter() - t0
+    forces, nl = model.forces(fused=True)
+    nl.buffer = 0.1 * max(float(f.r_cut[("A", "A")]) for f in forces[1:])
+    s = sim.Simulation(device=sim.GPU(0), seed=1)
+    s.create_state_from_snapshot(system.snapshot())
+    s.operations.integrator = sim.Integrator(
+        dt=0.003, forces=forces, methods=[sim.Langevin(kT=dna1.kT_from_temperature(296.15))])
+    s.run(max(warmup, 1))
+    dt = _timed_run(s, steps)
+    lib, ctx = s._lib, s._ctx
+    avg_ms = C.c_double()
+    assert lib.oxdna_time_forces(ctx, s._mask, 20, C.byref(avg_ms)) == 0
+    counts = (C.c_uint64 * 14)()
+    assert lib.oxdna_count_evaluations(ctx, s._mask, counts) == 0
+    census = census_all["oxdna1_fused" if model_name == "dna1" else "oxdna2_fused"]
+    flops = flops_per_launch(list(counts), census, evals_per_pair=2)
+    achieved = flops / (avg_ms.value * 1e-3) / 1e12
+    tpp = C.c_int()
+    assert lib.oxdna_get_tuning(ctx, C.byref(tpp), None) == 0
+    st = s.stats()
+    return {
+        "workload": f"{model_name}: 4096 x 64-bp duplexes, 524288 nt, box 640, skin {nl.buffer:.3f}, "
+                    "Langevin dt=0.003, fused",
+        "steps": steps, "steps_per_s": round(steps / dt, 1), "ms_per_step": round(dt * 1e3 / steps, 4),
+        "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 6), "traffic": None,
+                     "kernel": f"k_forces<{tpp.value}>", "kernel_ms": round(avg_ms.value, 4),
+                     "flops_per_launch": flops},
+        "pair_evaluations_per_s": round(sum(counts[2 * t] for t in range(6)) / 2 * steps / dt, 1),
+        "nlist_builds": st["nlist_builds"], "max_neigh": st["max_neigh"], "generate_s": round(t_gen, 1),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,9 +233,20 @@ def main():
                     help="replicas run one by one on the plain path for the comparison (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 256 MiB device copies (counter calibration for profiles)")
+    ap.add_argument("--large", action="store_true",
+                    help="only the 524,288-nt oxDNA2 and oxDNA1 systems (one JSON line)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.large:
+        with open(os.path.join(ROOT, "hoomd-oxdna_amd", "oxdna", "data", "census_flops.json")) as fh:
+            census_all = json.load(fh)
+        runs = [bench_large(m, args.steps, args.warmup, census_all) for m in ("dna2", "dna1")]
+        print(json.dumps({"metric": "oxDNA steps/s @524k nt", "value": runs[0]["steps_per_s"],
+                          "unit": "steps/s", "n_gpus": 1, "higher_is_better": True,
+                          "dtype": "fp32 (FP64 FENE, min-image, reductions, integrator state)",
+                          "data": "synthetic", "runs": runs}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the replica / "

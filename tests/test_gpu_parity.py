@@ -314,3 +314,33 @@ def test_stateless_drivers_match_oracle():
     assert np.abs(F[:, :3] - ref["F"]).max() <= F_REL * np.abs(ref["F"]).max() + F_ABS
     assert np.abs(T[:, :3] - ref["T"]).max() <= F_REL * np.abs(ref["T"]).max() + F_ABS
     assert np.allclose(S[:, :3].sum(axis=0), ref["E"][:3], rtol=E_REL, atol=E_PER_NT * n)
+
+
+@pytest.mark.parametrize("model", [dna2, dna1])
+def test_large_system_524k_nt(model):
+    """4096 ideal 64-bp duplexes (524,288 nt) in a box of 640: the FP32 engine against
+    the FP64 oracle over the whole system (per-term energies, every force and torque)."""
+    import time
+    t0 = time.time()
+    system = systems.duplex_gas(4096, 64, box=640.0, seed=3)
+    forces, nl = model.forces(fused=True)
+    nl.buffer = 0.1 * max(float(f.r_cut[("A", "A")]) for f in forces[1:])  # 10 % skin
+    _say(f"524k system built in {time.time() - t0:.1f} s; skin {nl.buffer:.3f}")
+    s = sim.Simulation(device=sim.GPU())
+    s.create_state_from_snapshot(system.snapshot())
+    s.operations.integrator = sim.Integrator(dt=0.003, forces=forces)
+    s.run(0)
+    e, f4, t3, _ = s._eval(s._mask)
+    ref = O.from_system(system, forces).compute()
+    n = system.n
+    rel = np.abs(e - ref["E"]) / np.maximum(np.abs(ref["E"]), 1e-300)
+    fmax, tmax = np.abs(ref["F"]).max(), np.abs(ref["T"]).max()
+    dF = np.abs(f4[:, :3] - ref["F"])
+    dT = np.abs(t3 - ref["T"])
+    frms = np.sqrt(np.mean(dF ** 2)) / np.sqrt(np.mean(ref["F"] ** 2))
+    _say(f"[524k {model.__name__.split('.')[-1]}] E/nt={np.round(ref['E'] / n, 6).tolist()} "
+         f"rel dE per term={np.round(rel, 9).tolist()} max dF/max|F|={dF.max() / fmax:.1e} "
+         f"rms dF/rms F={frms:.1e} max dT/max|T|={dT.max() / tmax:.1e}")
+    assert np.all(np.abs(e - ref["E"]) <= E_REL * np.abs(ref["E"]) + E_PER_NT * n)
+    assert dF.max() <= F_REL * fmax + F_ABS
+    assert dT.max() <= F_REL * tmax + F_ABS
