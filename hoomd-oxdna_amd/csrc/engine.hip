@@ -67,12 +67,19 @@ __global__ void k_zero_u32(uint32_t* p, size_t n) {
 // Rebuilds particle i's neighbour row inside the force kernel. The TPP lanes of the
 // particle split the (up to) 27 cells around it; each lane counts its matches (FP64
 // minimum-image distance, no contraction, bonded partners excluded), a lane prefix
-// sum places them in the LDS row, and a rank sort writes the row ordered by index to
-// LDS and HBM (deterministic for any order of the cell members). Lanes of one particle
-// are adjacent lanes of one wavefront; every thread of the block reaches the barriers.
+// sum places them in the LDS row, and a rank sort writes the row to LDS and HBM
+// (deterministic for any order of the cell members). Row entries are keys: the
+// neighbour index, with bit 31 set when the pair is beyond every cut-off but
+// Debye-Hueckel's even after the particles move by the skin; sorting by key puts the
+// near pairs first, so the lanes of later passes over the row all take the cheap
+// Debye-Hueckel-only path. Lanes of one particle are adjacent lanes of one wavefront;
+// every thread of the block reaches the barriers.
+constexpr uint32_t kFarBit = 0x80000000u;
+
+// 0: not a neighbour, 1: near neighbour, 2: far neighbour (Debye-Hueckel only).
 template<int TPP>
-__device__ __forceinline__ bool ref_pair(const NbrArgs& nb, const double4& pi, uint32_t i, uint32_t j, int b0, int b1) {
-    if (j == i || (int)j == b0 || (int)j == b1) return false;
+__device__ __forceinline__ int ref_pair(const NbrArgs& nb, const double4& pi, uint32_t i, uint32_t j, int b0, int b1) {
+    if (j == i || (int)j == b0 || (int)j == b1) return 0;
     const double4 pj = nb.pos[j];
     double dx = min_image(pi.x - pj.x, nb.L[0], nb.Linv[0]);
     double dy = min_image(pi.y - pj.y, nb.L[1], nb.Linv[1]);
@@ -84,7 +91,8 @@ __device__ __forceinline__ bool ref_pair(const NbrArgs& nb, const double4& pi, u
         r2 = r2 + dy * dy;
         r2 = r2 + dz * dz;
     }
-    return r2 < nb.rlist2;
+    if (!(r2 < nb.rlist2)) return 0;
+    return r2 < nb.rnear2 ? 1 : 2;
 }
 
 template<int TPP>
@@ -103,9 +111,10 @@ __device__ __forceinline__ uint32_t scan_cells(const NbrArgs& nb, const double4&
         uint32_t m = min(nb.count[cc], nb.cap);
         const uint32_t* mem = nb.members + (size_t)cc * nb.cap;
         for (uint32_t k = 0; k < m; ++k) {
-            uint32_t j = mem[k];
-            if (ref_pair<TPP>(nb, pi, i, j, b0, b1)) {
-                if (write && base + n < nb.max_neigh) row[base + n] = j;
+            const uint32_t j = mem[k];
+            const int cls = ref_pair<TPP>(nb, pi, i, j, b0, b1);
+            if (cls != 0) {
+                if (write && base + n < nb.max_neigh) row[base + n] = cls == 2 ? (j | kFarBit) : j;
                 ++n;
             }
         }
@@ -192,7 +201,7 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
         const int ti = __float_as_int(f1.w);
         const int nt = P.ntypes;
         for (uint32_t k = lane; k < nn; k += TPP) {
-            const uint32_t j = row[k];
+            const uint32_t j = row[k] & ~kFarBit;
             const double4 pj = a.pos[j];
             const float4 g0 = a.fr0[j], g1 = a.fr1[j], g2 = a.fr2[j];
             Frame fj{mk(g0.x, g0.y, g0.z), mk(g1.x, g1.y, g1.z), mk(g2.x, g2.y, g2.z)};
@@ -814,6 +823,11 @@ ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild) {
     nb.count = c->cell_count.p;
     nb.members = c->cell_members.p;
     nb.rlist2 = c->rlist * c->rlist;
+    // near: within the largest cut-off but Debye-Hueckel's, plus the skin
+    double rnear = 0.0;
+    for (int t = 0; t < 6; ++t)
+        if (t != T_DEBYE && ((mask >> t) & 1u)) rnear = std::max(rnear, c->rcut_max[t]);
+    nb.rnear2 = ((mask >> T_DEBYE) & 1u) ? (rnear + c->buffer) * (rnear + c->buffer) : nb.rlist2;
     nb.max_neigh = c->max_neigh;
     nb.pos_last = c->pos_last.p;
     // rebuild: 0 never (rows are current), 1 always, 2 when the skin test asked for it
@@ -1474,14 +1488,19 @@ int oxdna_get_nlist_pairs(oxdna_ctx* c, int32_t* pairs, size_t max_pairs, size_t
                               c->stream));
         HIP_OK(hipStreamSynchronize(c->stream));
         size_t cnt = 0;
-        for (uint32_t i = 0; i < n; ++i)
+        std::vector<uint32_t> js;
+        for (uint32_t i = 0; i < n; ++i) {
+            js.clear();
             for (uint32_t k = 0; k < nn[i]; ++k) {
-                uint32_t j = rows[(size_t)i * c->max_neigh + k];
-                if (j > i) {
-                    if (cnt < max_pairs) { pairs[2 * cnt] = (int32_t)i; pairs[2 * cnt + 1] = (int32_t)j; }
-                    ++cnt;
-                }
+                uint32_t j = rows[(size_t)i * c->max_neigh + k] & 0x7fffffffu;  // drop the far-pair bit
+                if (j > i) js.push_back(j);
             }
+            std::sort(js.begin(), js.end());  // rows hold near pairs first
+            for (uint32_t j : js) {
+                if (cnt < max_pairs) { pairs[2 * cnt] = (int32_t)i; pairs[2 * cnt + 1] = (int32_t)j; }
+                ++cnt;
+            }
+        }
         if (npairs) *npairs = cnt;
         if (cnt > max_pairs) rc = fail(c, "oxdna_get_nlist_pairs: buffer too small");
     });
