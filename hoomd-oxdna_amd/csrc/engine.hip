@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
     const DParams& P = *a.P;
     const uint32_t mask = a.mask;
     const NbrArgs& nb = a.nb;
-    const bool rebuild = nb.force_rebuild || (nb.flags && nb.flags->rebuild);
+    const bool rebuild = nb.force_rebuild || (nb.rebuild_flag && *nb.rebuild_flag);
 
     uint32_t nn = 0;
     const uint32_t* row = nullptr;
@@ -163,7 +163,8 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
         if (active) {
             for (uint32_t e = lane; e < m; e += TPP) {
                 uint32_t x = urow[e], r = 0;
-                for (uint32_t q = 0; q < m; ++q) r += urow[q] < x ? 1u : 0u;
+                // ties (never expected) break by position, so every rank is distinct
+                for (uint32_t q = 0; q < m; ++q) r += (urow[q] < x || (urow[q] == x && q < e)) ? 1u : 0u;
                 srow[r] = x;
                 a.nlist[a.head[i] + r] = x;
             }
@@ -299,6 +300,14 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
                 a.virial[s * a.vpitch + i] = v;
             }
         }
+        if (a.integrate) {
+            // second half of the Langevin step with the force just computed (touches
+            // only particle i's momenta, which no other particle's evaluation reads)
+            double4 v = a.vel[i], p = a.angmom[i], bd, bdt;
+            lv_second_half(a.lv, i, a.sflags->step, a.seed, fo.x, fo.y, fo.z, to.x, to.y, to.z, a.quat[i], v, p,
+                           bd, bdt);
+            a.vel[i] = v; a.angmom[i] = p; a.bd[i] = bd; a.bd_tq[i] = bdt;
+        }
     }
     if (WANT_E) {
         // per-term totals: FP64 from here on (wave shuffle, then across waves in LDS)
@@ -342,6 +351,9 @@ __global__ void k_reduce_energy(uint32_t nblocks, const double* __restrict__ par
 __device__ __forceinline__ uint32_t step1_particle(const StepArgs& a, uint32_t i) {
     const Langevin lv{a.dt, a.kT, a.gamma, a.gamma_r, a.mass, a.inertia};
     double4 x = a.pos[i], v = a.vel[i], q = a.quat[i], p = a.angmom[i];
+    // the cell this particle was binned into last step (other parity): clear it for
+    // the step after this one (nobody reads or writes that buffer during this step)
+    if (a.cells.count_prev) a.cells.count_prev[cell_index(x.x, x.y, x.z, a.cells.Linv, a.cells.nc)] = 0u;
     const float4 f = a.force[i], t = a.torque[i];
     lv_first_half(lv, a.L, a.Linv, f.x, f.y, f.z, t.x, t.y, t.z, a.bd[i], a.bd_tq[i], x, v, q, p);
     a.pos[i] = x; a.vel[i] = v; a.quat[i] = q; a.angmom[i] = p;
@@ -357,35 +369,25 @@ __device__ __forceinline__ uint32_t step1_particle(const StepArgs& a, uint32_t i
         double dx = min_image(x.x - l.x, a.L[0], a.Linv[0]);
         double dy = min_image(x.y - l.y, a.L[1], a.Linv[1]);
         double dz = min_image(x.z - l.z, a.L[2], a.Linv[2]);
-        if (dx * dx + dy * dy + dz * dz > a.skin_half2) atomicOr(&a.flags->rebuild, 1u);
+        if (dx * dx + dy * dy + dz * dz > a.skin_half2) atomicOr(a.rebuild_set, 1u);
         return occ;
     }
     return 0;
 }
 
 // First half of the step (lv_first_half), then the new frames, cell binning, and the
-// skin test that requests a neighbour-list rebuild.
+// skin test that requests a neighbour-list rebuild. The second half runs in the force
+// kernel's epilogue. Cell counts and rebuild requests alternate between two buffers by
+// step parity, so that one kernel never clears what another workgroup of it reads.
 __global__ void k_step1(StepArgs a) {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i == 0) a.flags->step += 1ull;
+    if (i == 0) {
+        a.flags->step += 1ull;
+        *a.rebuild_clear = 0u;
+    }
     uint32_t occ = 0;
     if (i < a.N) occ = step1_particle(a, i);
     if (a.cells.count) wave_atomic_max(&a.flags->max_cell, occ);
-}
-
-// Second half (lv_second_half); also clears this particle's cell count for the next
-// step's binning.
-__global__ void k_step2(StepArgs a) {
-    uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i == 0) a.flags->rebuild = 0u;
-    if (i >= a.N) return;
-    const Langevin lv{a.dt, a.kT, a.gamma, a.gamma_r, a.mass, a.inertia};
-    const double4 x = a.pos[i];
-    if (a.cells.count) a.cells.count[cell_index(x.x, x.y, x.z, a.cells.Linv, a.cells.nc)] = 0u;
-    double4 v = a.vel[i], p = a.angmom[i], bd, bdt;
-    const float4 f = a.force[i], t = a.torque[i];
-    lv_second_half(lv, i, a.flags->step, a.seed, f.x, f.y, f.z, t.x, t.y, t.z, a.quat[i], v, p, bd, bdt);
-    a.vel[i] = v; a.angmom[i] = p; a.bd[i] = bd; a.bd_tq[i] = bdt;
 }
 
 }  // namespace oxd
@@ -713,6 +715,9 @@ void check_force_args(const ForceArgs& a) {
     if ((a.mask & (1u << T_BONDED)) && (!a.bpart || !a.binfo)) throw Fail("internal: bonded launch without bond table");
     if (a.nb.flags && (!a.nb.count || !a.nb.members || !a.nb.pos_last || !a.bpart))
         throw Fail("internal: row rebuild without a cell list");
+    if (a.nb.rebuild_flag && !a.nb.flags) throw Fail("internal: rebuild request without a flags block");
+    if (a.integrate && (!a.sflags || !a.quat || !a.vel || !a.angmom || !a.bd || !a.bd_tq))
+        throw Fail("internal: fused Langevin half-kick without its arrays");
 }
 void launch_k_any(int tpp, bool e, bool v, dim3 g, dim3 b, size_t lds, hipStream_t st, const ForceArgs& a) {
     check_force_args(a);
@@ -744,10 +749,10 @@ void setup_cells(oxdna_ctx* c) {
         double per_cell = (double)c->N / (double)total;
         c->cell_cap = (uint32_t)std::max(16.0, std::ceil(per_cell * 4.0 + 8.0));
     }
-    c->cell_count.alloc(total);
+    c->cell_count.alloc(2 * total);  // two buffers, alternating by step parity
     c->cell_members.alloc(total * c->cell_cap);
-    hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for(total), 4096)), dim3(kBlock), 0, c->stream,
-                       c->cell_count.p, total);
+    hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for(2 * total), 4096)), dim3(kBlock), 0, c->stream,
+                       c->cell_count.p, 2 * total);
     if (c->max_neigh == 0) c->max_neigh = 64;
     if (c->max_neigh > kMaxNeigh) throw Fail("neighbour rows would exceed " + std::to_string(kMaxNeigh) + " entries");
     c->nlist.alloc((size_t)c->N * c->max_neigh);
@@ -761,13 +766,17 @@ void setup_cells(oxdna_ctx* c) {
 
 size_t ncell_total(const oxdna_ctx* c) { return (size_t)c->nc[0] * c->nc[1] * c->nc[2]; }
 
-CellArgs cell_args(oxdna_ctx* c, bool with_cells) {
+// Cells of a step of parity `parity` (the counts buffer it bins into; the other one
+// is cleared as the particles leave it).
+CellArgs cell_args(oxdna_ctx* c, bool with_cells, int parity) {
     CellArgs a;
     a.N = c->N;
     a.pos = c->pos.p;
     for (int k = 0; k < 3; ++k) { a.Linv[k] = c->Linv[k]; a.nc[k] = c->nc[k]; }
     a.cap = c->cell_cap;
-    a.count = with_cells ? c->cell_count.p : nullptr;
+    const size_t nct = (size_t)c->nc[0] * c->nc[1] * c->nc[2];
+    a.count = with_cells ? c->cell_count.p + (size_t)parity * nct : nullptr;
+    a.count_prev = with_cells ? c->cell_count.p + (size_t)(parity ^ 1) * nct : nullptr;
     a.members = c->cell_members.p;
     a.flags = c->flags.p;
     return a;
@@ -781,7 +790,7 @@ void read_flags(oxdna_ctx* c, DevFlags& f) {
 void clear_overflow(oxdna_ctx* c) {
     DevFlags f;
     read_flags(c, f);
-    f.overflow = 0; f.max_cell = 0; f.max_neigh = 0; f.rebuild = 0;
+    f.overflow = 0; f.max_cell = 0; f.max_neigh = 0; f.rebuild[0] = f.rebuild[1] = 0;
     HIP_OK(hipMemcpyAsync(c->flags.p, &f, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
 }
 
@@ -795,8 +804,11 @@ void ensure_ready(oxdna_ctx* c) {
     upload_params(c);
 }
 
-ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild) {
-    ForceArgs a;
+// rebuild: 0 never (rows are current), 1 always, 2 when the skin test of the step of
+// parity `parity` asked for it; integrate: fuse the Langevin second half-kick.
+ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild, int parity = 0,
+                     bool integrate = false) {
+    ForceArgs a{};
     a.N = c->N;
     a.pos = c->pos.p;
     a.fr0 = c->fr0.p; a.fr1 = c->fr1.p; a.fr2 = c->fr2.p;
@@ -820,7 +832,7 @@ ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild) {
     nb.pos = c->pos.p;
     for (int k = 0; k < 3; ++k) { nb.L[k] = c->L[k]; nb.Linv[k] = c->Linv[k]; nb.nc[k] = c->nc[k]; }
     nb.cap = c->cell_cap;
-    nb.count = c->cell_count.p;
+    nb.count = c->cell_count.p + (size_t)parity * c->nc[0] * c->nc[1] * c->nc[2];
     nb.members = c->cell_members.p;
     nb.rlist2 = c->rlist * c->rlist;
     // near: within the largest cut-off but Debye-Hueckel's, plus the skin
@@ -830,16 +842,26 @@ ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild) {
     nb.rnear2 = ((mask >> T_DEBYE) & 1u) ? (rnear + c->buffer) * (rnear + c->buffer) : nb.rlist2;
     nb.max_neigh = c->max_neigh;
     nb.pos_last = c->pos_last.p;
-    // rebuild: 0 never (rows are current), 1 always, 2 when the skin test asked for it
     nb.flags = (rebuild != 0 && (mask & 0x3fu)) ? c->flags.p : nullptr;
+    nb.rebuild_flag = (rebuild == 2 && (mask & 0x3fu)) ? &c->flags.p->rebuild[parity] : nullptr;
     nb.force_rebuild = (rebuild == 1 && (mask & 0x3fu)) ? 1 : 0;
+    a.integrate = integrate ? 1 : 0;
+    a.lv = Langevin{c->dt, c->kT, c->gamma, c->gamma_r, 1.0, 1.0};
+    a.seed = c->seed;
+    a.sflags = c->flags.p;
+    a.quat = c->quat.p;
+    a.vel = c->vel.p;
+    a.angmom = c->angmom.p;
+    a.bd = c->bd.p;
+    a.bd_tq = c->bd_tq.p;
     return a;
 }
 
 uint32_t force_blocks(const oxdna_ctx* c, int tpp) { return blocks_for((size_t)c->N * tpp); }
 
-void launch_forces(oxdna_ctx* c, uint32_t mask, int accumulate, bool want_e, bool want_v, int rebuild) {
-    ForceArgs a = force_args(c, mask, accumulate, rebuild);
+void launch_forces(oxdna_ctx* c, uint32_t mask, int accumulate, bool want_e, bool want_v, int rebuild,
+                   int parity = 0, bool integrate = false) {
+    ForceArgs a = force_args(c, mask, accumulate, rebuild, parity, integrate);
     dim3 grid(force_blocks(c, c->tpp)), block(kBlock);
     size_t lds = a.nb.flags ? lds_bytes(c, c->tpp) : 0;
     launch_k_any(c->tpp, want_e, want_v, grid, block, lds, c->stream, a);
@@ -856,7 +878,12 @@ void compute_rebuild(oxdna_ctx* c, uint32_t m, int fused) {
     for (int attempt = 0; attempt < 8; ++attempt) {
         if (pairs) {
             clear_overflow(c);
-            hipLaunchKernelGGL(k_cell_assign, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, cell_args(c, true));
+            // a run leaves its last step's counts in one buffer: start from empty cells
+            size_t nct2 = 2 * ncell_total(c);
+            hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for(nct2), 4096)), dim3(kBlock), 0,
+                               c->stream, c->cell_count.p, nct2);
+            hipLaunchKernelGGL(k_cell_assign, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream,
+                               cell_args(c, true, 0));
         }
         if (fused || m == 0) {
             launch_forces(c, m, 0, true, true, pairs ? 1 : 0);
@@ -873,7 +900,7 @@ void compute_rebuild(oxdna_ctx* c, uint32_t m, int fused) {
             c->nlist_valid = true;
             return;
         }
-        size_t nct = ncell_total(c);
+        size_t nct = 2 * ncell_total(c);  // both parities start the next step empty
         hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for(nct), 4096)), dim3(kBlock), 0, c->stream,
                            c->cell_count.p, nct);
         DevFlags f;
@@ -908,7 +935,7 @@ void compute(oxdna_ctx* c, uint32_t mask, int fused) {
     c->forces_mask = m;
 }
 
-StepArgs step_args(oxdna_ctx* c, bool with_cells) {
+StepArgs step_args(oxdna_ctx* c, bool with_cells, int parity) {
     StepArgs a;
     a.N = c->N;
     a.bd = c->bd.p; a.bd_tq = c->bd_tq.p;
@@ -921,18 +948,21 @@ StepArgs step_args(oxdna_ctx* c, bool with_cells) {
     a.skin_half2 = 0.25 * c->buffer * c->buffer;
     a.seed = c->seed;
     a.flags = c->flags.p;
-    a.cells = cell_args(c, with_cells);
+    a.rebuild_set = &c->flags.p->rebuild[parity];
+    a.rebuild_clear = &c->flags.p->rebuild[parity ^ 1];
+    a.cells = cell_args(c, with_cells, parity);
     return a;
 }
 
-// One step: step1 (kick, drift, rotate, frames, binning, skin test) -> forces (rows
-// rebuilt in-kernel when the skin test fired) -> step2 (Langevin kick, cell reset).
-void enqueue_step(oxdna_ctx* c, uint32_t mask) {
-    bool pairs = (mask & 0x3fu) != 0;
-    StepArgs s = step_args(c, pairs);
+// One step, number `step` (1-based): step1 (kick, drift, rotate, frames, binning, skin
+// test) -> forces (rows rebuilt in-kernel when the skin test fired; Langevin second
+// half-kick in the epilogue). Buffers alternate with the step's parity.
+void enqueue_step(oxdna_ctx* c, uint32_t mask, uint64_t step) {
+    const bool pairs = (mask & 0x3fu) != 0;
+    const int parity = (int)(step & 1u);
+    StepArgs s = step_args(c, pairs, parity);
     hipLaunchKernelGGL(k_step1, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, s);
-    launch_forces(c, mask, 0, false, false, pairs ? 2 : 0);
-    hipLaunchKernelGGL(k_step2, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, s);
+    launch_forces(c, mask, 0, false, false, pairs ? 2 : 0, parity, true);
 }
 
 constexpr int kGraphSteps = 16;
@@ -940,8 +970,10 @@ constexpr int kGraphSteps = 16;
 void build_graph(oxdna_ctx* c, uint32_t mask) {
     c->drop_graph();
     hipGraph_t g;
+    // replayed only when an even number of steps is done: step k of the graph is step
+    // number done + 1 + k, of parity (1 + k) & 1
     HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int s = 0; s < kGraphSteps; ++s) enqueue_step(c, mask);
+    for (int s = 0; s < kGraphSteps; ++s) enqueue_step(c, mask, (uint64_t)s + 1);
     HIP_OK(hipStreamEndCapture(c->stream, &g));
     HIP_OK(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
     HIP_OK(hipGraphDestroy(g));
@@ -1025,14 +1057,23 @@ void run(oxdna_ctx* c, uint32_t nsteps, uint32_t mask, int use_graph) {
         unsigned long long step0 = f0.step;
         for (int attempt = 0;; ++attempt) {
             uint32_t left = chunk;
+            uint64_t step = step0;  // steps done so far
             if (use_graph) {
                 if (!c->graph || c->graph_mask != m) build_graph(c, m);
+                if ((step & 1u) && left > 0) {  // the graph starts at an even count
+                    enqueue_step(c, m, ++step);
+                    --left;
+                }
                 while (left >= (uint32_t)c->graph_steps) {
                     HIP_OK(hipGraphLaunch(c->graph, c->stream));
                     left -= c->graph_steps;
+                    step += c->graph_steps;
                 }
             }
-            while (left > 0) { enqueue_step(c, m); --left; }
+            while (left > 0) {
+                enqueue_step(c, m, ++step);
+                --left;
+            }
             DevFlags f;
             read_flags(c, f);
             if (f.overflow == 0) break;
@@ -1041,7 +1082,7 @@ void run(oxdna_ctx* c, uint32_t nsteps, uint32_t mask, int use_graph) {
             restore_state(c, saved);
             c->max_neigh = std::max(c->max_neigh, (uint32_t)std::ceil(f.max_neigh * 1.5) + 8);
             c->cell_cap = std::max(c->cell_cap, (uint32_t)std::ceil(f.max_cell * 1.5) + 4);
-            f.overflow = 0; f.step = step0; f.rebuild = 0;
+            f.overflow = 0; f.step = step0; f.rebuild[0] = f.rebuild[1] = 0;
             HIP_OK(hipMemcpyAsync(c->flags.p, &f, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
             c->drop_graph();
             setup_cells(c);

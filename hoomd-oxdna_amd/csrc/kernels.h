@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "integrator.h"
 #include "oxdna_device.h"
 
 namespace oxd {
@@ -15,7 +16,7 @@ enum { S_FENE = 0, S_BEXC = 1, S_STACK = 2, S_NEXC = 3, S_HB = 4, S_CR = 5, S_CX
 
 // Flags / counters shared by the step kernels (device memory, one per context).
 struct DevFlags {
-    uint32_t rebuild;        // neighbour rows must be rebuilt this step
+    uint32_t rebuild[2];     // neighbour rows must be rebuilt this step (by step parity)
     uint32_t overflow;       // bit0: a cell overflowed, bit1: a neighbour row overflowed
     uint32_t max_cell;       // largest cell occupancy seen
     uint32_t max_neigh;      // largest neighbour count seen
@@ -126,6 +127,7 @@ struct CellArgs {
     int nc[3];
     uint32_t cap;
     uint32_t* count;    // NULL: no cell list (no nonbonded term)
+    uint32_t* count_prev;  // step kernels: the other step parity's counts (zeroed as they go)
     uint32_t* members;
     DevFlags* flags;
 };
@@ -170,7 +172,8 @@ struct NbrArgs {
     uint32_t max_neigh;
     double4* pos_last;
     DevFlags* flags;       // NULL: never rebuild (rows supplied by the caller)
-    int force_rebuild;     // rebuild regardless of flags->rebuild
+    const uint32_t* rebuild_flag;  // this step's rebuild request (NULL: none)
+    int force_rebuild;     // rebuild regardless of the request
 };
 
 struct ForceArgs {
@@ -196,6 +199,16 @@ struct ForceArgs {
     double4* split;        // optional per-particle (fene, bexc, stack) half energies (WANT_E)
     unsigned long long* counts;  // COUNT instantiation: per term (inside cut-off, nonzero)
     NbrArgs nb;
+    // second half of the Langevin step, fused into the force kernel's epilogue
+    int integrate;
+    Langevin lv;
+    unsigned long long seed;
+    const DevFlags* sflags;  // step counter (random stream)
+    const double4* quat;
+    double4* vel;
+    double4* angmom;
+    double4* bd;
+    double4* bd_tq;
 };
 
 struct StepArgs {
@@ -218,6 +231,8 @@ struct StepArgs {
     double skin_half2;
     unsigned long long seed;
     DevFlags* flags;
+    uint32_t* rebuild_set;    // this step's rebuild request
+    uint32_t* rebuild_clear;  // the other parity's request, cleared for the next step
     CellArgs cells;
 };
 
