@@ -537,7 +537,15 @@ uint32_t blocks_for(size_t threads) { return (uint32_t)((threads + kBlock - 1) /
 
 // ------------------------------------------------------------------ parameter conversion
 void cvt3(float d[3], const double s[3]) { d[0] = (float)s[0]; d[1] = (float)s[1]; d[2] = (float)s[2]; }
-F4P cvt_f4(const double s[5]) { return F4P{(float)s[0], (float)s[1], (float)s[2], (float)s[3], (float)s[4]}; }
+// f4 set (a, b, theta0, theta_s, theta_c) plus the open interval of cos(theta) on
+// which f4 is nonzero: |theta - theta0| < theta_c  <=>  cos(theta0 + theta_c) < c <
+// cos(theta0 - theta_c), either side unbounded where the angle range hits 0 or pi.
+F4P cvt_f4(const double s[5]) {
+    const double lo_th = s[2] - s[4], hi_th = s[2] + s[4];
+    const float clo = hi_th >= M_PI ? -2.0f : (float)std::cos(hi_th);
+    const float chi = lo_th <= 0.0 ? 2.0f : (float)std::cos(lo_th);
+    return F4P{(float)s[0], (float)s[1], (float)s[2], (float)s[3], (float)s[4], clo, chi};
+}
 F5P cvt_f5(const double s[4]) { return F5P{(float)s[0], (float)s[1], (float)s[2], (float)s[3]}; }
 
 template<class T> void require_uniform(const T* tab, int nt, const char* term, T (*clear)(T)) {

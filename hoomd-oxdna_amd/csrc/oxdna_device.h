@@ -59,7 +59,7 @@ __device__ __forceinline__ float acos_as(float x, float& sin_out) {
 __device__ __forceinline__ float clampc(float c) { return fminf(1.0f, fmaxf(-1.0f, c)); }
 
 // ------------------------------------------------------------------ parameter blocks (FP32)
-struct F4P { float a, b, t0, ts, tc; };    // angular modulation set
+struct F4P { float a, b, t0, ts, tc, clo, chi; };  // angular modulation set; f4 != 0 iff clo < cos < chi
 struct F5P { float a, b, xc, xs; };        // dihedral set
 struct F1P { float a, r0, rlow, rhigh, rclow, rchigh, blow, bhigh; };
 struct F2P { float k, r0, rc, rlow, rhigh, rclow, rchigh, blow, bhigh; };
@@ -182,6 +182,17 @@ __device__ __forceinline__ void f5_vd(float c, const F5P& p, float& v, float& d)
     }
 }
 
+// Whether the angular factor of cosine c can be nonzero (the reflected angle too when
+// sym): a comparison in cosine space that spares the arccos of screened-out pairs.
+// Slack of a few ulps keeps every pair whose f4 the full evaluation finds nonzero.
+__device__ __forceinline__ bool f4_nz(float c, const F4P& p, bool sym) {
+    const float slack = 4e-6f;
+    c = clampc(c);
+    bool nz = c > p.clo - slack && c < p.chi + slack;
+    if (sym) nz = nz || (-c > p.clo - slack && -c < p.chi + slack);
+    return nz;
+}
+
 // Angular factor with optional reflection (reference src/OxDNABaseBase.h:52-67):
 // value f4(th) [+ f4(pi - th)], coefficient sign * (D(th) [- D(pi - th)]).
 struct Ang { float v, d; };
@@ -261,6 +272,10 @@ __device__ __forceinline__ float bb_term(const BB& g, const Frame& fi, const Fra
                                          const F4P& t1, const F4P& t23, const F4P& t4, const F4P& t78,
                                          bool sym, bool use_t4, V& F, V& T) {
     if (rv == 0.f) return 0.f;  // zero radial => zero energy, no force (early exit of the reference)
+    // a zero angular factor makes the term zero: screen all six in cosine space first
+    if (!(f4_nz(g.c1, t1, false) && f4_nz(g.c2, t23, false) && f4_nz(g.c3, t23, false) &&
+          (!use_t4 || f4_nz(g.c4, t4, sym)) && f4_nz(g.c7, t78, sym) && f4_nz(g.c8, t78, sym)))
+        return 0.f;
     Ang A1 = angular(g.c1, t1, false, -1.f);
     Ang A2 = angular(g.c2, t23, false, -1.f);
     Ang A3 = angular(g.c3, t23, false, 1.f);
@@ -305,6 +320,8 @@ __device__ __forceinline__ float coax_pair(V Rc, const Frame& fi, const Frame& f
     if (fr == 0.f) return 0.f;
     V rsd = rs * (1.f / rsm);
     float c1 = -dot(fi.a1, fj.a1), c4 = dot(fi.a3, fj.a3), c5 = dot(fi.a3, rsd), c6 = -dot(fj.a3, rsd);
+    // theta4, theta5 and theta6 factors screened in cosine space (theta1 has extra modes)
+    if (!(f4_nz(c4, p.t4, false) && f4_nz(c5, p.t56, true) && f4_nz(c6, p.t56, true))) return 0.f;
     float s1, s4, s5, s6;
     float th1 = acos_as(clampc(c1), s1), th4 = acos_as(clampc(c4), s4);
     float th5 = acos_as(clampc(c5), s5), th6 = acos_as(clampc(c6), s6);
