@@ -475,6 +475,7 @@ struct oxdna_ctx {
     int tpp = 16;                        // force-kernel lanes per particle (autotuned)
     uint32_t tuned_mask = 0xffffffffu;
     uint32_t tuned_n = 0;
+    int fixed_tpp = 0;                   // nonzero: lanes per particle pinned by the caller
     uint32_t forces_mask = 0xffffffffu;  // mask of the forces currently in `force`
     hipGraphExec_t graph = nullptr;
     uint32_t graph_mask = 0;
@@ -992,6 +993,15 @@ void build_graph(oxdna_ctx* c, uint32_t mask) {
 // Autotuner: lanes per particle (8, 16, 32) of the force kernel, timed at the
 // current state (rows current, no rebuild) with HIP events; keeps the fastest.
 void autotune(oxdna_ctx* c, uint32_t m) {
+    if (c->fixed_tpp) {
+        // the pinned value, unless the rows grew too long for its LDS (then 32 lanes)
+        const int want = lds_bytes(c, c->fixed_tpp) > kMaxLds ? 32 : c->fixed_tpp;
+        if (c->tpp != want) {
+            c->tpp = want;
+            c->drop_graph();
+        }
+        return;
+    }
     if (c->tuned_mask == m && c->tuned_n == c->N) return;
     const int cand[3] = {8, 16, 32};
     float best = 1e30f;
@@ -1797,6 +1807,20 @@ int oxdna_get_stats(oxdna_ctx* c, uint64_t* n_builds, uint32_t* max_neigh, uint6
         if (n_builds) *n_builds = f.builds;
         if (max_neigh) *max_neigh = c->max_neigh;
         if (step) *step = c->n_rep ? c->host_step : f.step;
+    });
+}
+
+int oxdna_set_lanes_per_particle(oxdna_ctx* c, int lanes) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (lanes != 0 && lanes != 8 && lanes != 16 && lanes != 32)
+            throw Fail("oxdna_set_lanes_per_particle: 0 (autotune), 8, 16 or 32");
+        if (lanes != 0 && lds_bytes(c, lanes) > kMaxLds)
+            throw Fail("oxdna_set_lanes_per_particle: neighbour rows too long for that many particles per block");
+        c->fixed_tpp = lanes;
+        if (lanes) c->tpp = lanes;
+        c->tuned_mask = 0xffffffffu;
+        c->drop_graph();
     });
 }
 

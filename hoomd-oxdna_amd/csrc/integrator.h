@@ -42,10 +42,16 @@ __device__ __forceinline__ void to_lab(double4 q, double vx, double vy, double v
     double4 t = qmul(qmul(q, make_double4(0.0, vx, vy, vz)), qconj(q));
     bx = t.y; by = t.z; bz = t.w;
 }
-// cos and sin of x: Taylor series through x^13 for |x| <= 1/4 (truncation below
-// 2^-60, i.e. exact in FP64), the library functions otherwise.
+// cos and sin of x: Taylor series through x^7 / x^6 for |x| <= 2^-10 (the first
+// dropped terms, x^9/9! and x^8/8!, are below 2^-100 relative: exact in FP64), through
+// x^13 for |x| <= 1/4 (truncation below 2^-60), the library functions otherwise. A
+// Langevin rotor turns by |omega| dt / 2 ~ 1e-3 rad, so the short series is the rule.
 __device__ __forceinline__ void sincos_small(double x, double& s, double& c) {
-    if (fabs(x) <= 0.25) {
+    if (fabs(x) <= 0.0009765625) {
+        double x2 = x * x;
+        s = x * (1.0 + x2 * (-1.0 / 6 + x2 * (1.0 / 120 + x2 * (-1.0 / 5040))));
+        c = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24 + x2 * (-1.0 / 720)));
+    } else if (fabs(x) <= 0.25) {
         double x2 = x * x;
         s = x * (1.0 + x2 * (-1.0 / 6 + x2 * (1.0 / 120 + x2 * (-1.0 / 5040 + x2 * (1.0 / 362880 +
                  x2 * (-1.0 / 39916800 + x2 * (1.0 / 6227020800.0)))))));
@@ -62,9 +68,10 @@ __device__ __forceinline__ double4 permute(int k, double4 q) {
     if (k == 2) return make_double4(-q.z, -q.w, q.x, q.y);
     return make_double4(-q.w, q.z, -q.y, q.x);
 }
-__device__ __forceinline__ void rotor(int k, double tau, double Ik, double4& q, double4& p) {
+// inv4I = 1 / (4 I_k).
+__device__ __forceinline__ void rotor(int k, double tau, double inv4I, double4& q, double4& p) {
     double4 Pq = permute(k, q), Pp = permute(k, p);
-    double zeta = (p.x * Pq.x + p.y * Pq.y + p.z * Pq.z + p.w * Pq.w) / (4.0 * Ik);
+    double zeta = (p.x * Pq.x + p.y * Pq.y + p.z * Pq.z + p.w * Pq.w) * inv4I;
     double s, c;
     sincos_small(zeta * tau, s, c);
     q = make_double4(c * q.x + s * Pq.x, c * q.y + s * Pq.y, c * q.z + s * Pq.z, c * q.w + s * Pq.w);
@@ -95,12 +102,12 @@ __device__ __forceinline__ void lv_first_half(const Langevin& lv, const double L
     to_body(q, tx + bdt.x, ty + bdt.y, tz + bdt.z, tbx, tby, tbz);
     double4 kick = qmul(q, make_double4(0.0, tbx, tby, tbz));
     p.x += dt * kick.x; p.y += dt * kick.y; p.z += dt * kick.z; p.w += dt * kick.w;
-    const double I = lv.inertia;
-    rotor(3, 0.5 * dt, I, q, p);
-    rotor(2, 0.5 * dt, I, q, p);
-    rotor(1, dt, I, q, p);
-    rotor(2, 0.5 * dt, I, q, p);
-    rotor(3, 0.5 * dt, I, q, p);
+    const double inv4I = 0.25 / lv.inertia;
+    rotor(3, 0.5 * dt, inv4I, q, p);
+    rotor(2, 0.5 * dt, inv4I, q, p);
+    rotor(1, dt, inv4I, q, p);
+    rotor(2, 0.5 * dt, inv4I, q, p);
+    rotor(3, 0.5 * dt, inv4I, q, p);
     double nq = 1.0 / sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
     q.x *= nq; q.y *= nq; q.z *= nq; q.w *= nq;
 }

@@ -40,6 +40,7 @@ _SIGS = {
     "oxdna_get_state": (C.c_int, [_p, _dp, _dp, _dp, _dp]),
     "oxdna_synchronize": (C.c_int, [_p]),
     "oxdna_get_stats": (C.c_int, [_p, C.POINTER(_u64), C.POINTER(_u32), C.POINTER(_u64)]),
+    "oxdna_set_lanes_per_particle": (C.c_int, [_p, C.c_int]),
     "oxdna_get_tuning": (C.c_int, [_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "oxdna_time_forces": (C.c_int, [_p, _u32, _u32, C.POINTER(C.c_double)]),
     "oxdna_count_evaluations": (C.c_int, [_p, _u32, C.POINTER(_u64)]),

@@ -127,7 +127,10 @@ class _State:
 class Simulation:
     """One system on one GPU, driven through the engine's C ABI."""
 
-    def __init__(self, device=None, seed=0, use_graph=True):
+    def __init__(self, device=None, seed=0, use_graph=True, lanes_per_particle=0):
+        """lanes_per_particle: 8, 16 or 32 lanes of the force kernel per particle, or 0
+        to autotune (the choice fixes the summation order: pin it for bit-reproducible
+        runs across processes)."""
         self.device = device if device is not None else GPU()
         if not isinstance(self.device, GPU):
             raise RuntimeError("the oxDNA HIP engine runs on a GPU device only")
@@ -137,6 +140,9 @@ class Simulation:
         self._ctx = C.c_void_p()
         _engine.check(self._lib, None, self._lib.oxdna_ctx_create(self.device.gpu_id, None,
                                                                   C.byref(self._ctx)), "ctx_create")
+        if lanes_per_particle:
+            self._check(self._lib.oxdna_set_lanes_per_particle(self._ctx, int(lanes_per_particle)),
+                        "set_lanes_per_particle")
         self._snap = None
         self._attached = False
         self._mask = 0

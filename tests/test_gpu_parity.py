@@ -170,8 +170,8 @@ def test_unfused_compute_matches_fused():
     assert np.allclose(f1, f2, rtol=1e-5, atol=1e-5)
 
 
-def _langevin_sim(system, forces, seed=5, use_graph=True, kT=None):
-    s = sim.Simulation(device=sim.GPU(), seed=seed, use_graph=use_graph)
+def _langevin_sim(system, forces, seed=5, use_graph=True, kT=None, lanes=0):
+    s = sim.Simulation(device=sim.GPU(), seed=seed, use_graph=use_graph, lanes_per_particle=lanes)
     s.create_state_from_snapshot(system.snapshot())
     kT = dna1.kT_from_temperature(296.15) if kT is None else kT
     s.operations.integrator = sim.Integrator(dt=0.003, forces=forces, methods=[sim.Langevin(kT=kT)])
@@ -192,11 +192,13 @@ def test_duplex_stays_bound_langevin():
 
 
 def test_graph_and_direct_steps_agree():
+    """With the lanes per particle pinned (the summation order), graph replays and direct
+    steps give bit-identical trajectories."""
     system = H.cfg1_system(4)
     out = []
     for g in (True, False):
         forces, _ = dna2.forces(fused=True)
-        s = _langevin_sim(system, forces, seed=9, use_graph=g)
+        s = _langevin_sim(system, forces, seed=9, use_graph=g, lanes=16)
         s.run(100)
         out.append(s.state.get_snapshot().particles.position.copy())
     _say("graph vs direct max |dx|", np.abs(out[0] - out[1]).max())
