@@ -390,6 +390,18 @@ __global__ void k_step1(StepArgs a) {
     if (a.cells.count) wave_atomic_max(&a.flags->max_cell, occ);
 }
 
+// Second half of the step (lv_second_half), as its own kernel (systems large enough
+// to fill the GPU; smaller ones run it in the force kernel's epilogue).
+__global__ void k_step2(StepArgs a) {
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= a.N) return;
+    const Langevin lv{a.dt, a.kT, a.gamma, a.gamma_r, a.mass, a.inertia};
+    double4 v = a.vel[i], p = a.angmom[i], bd, bdt;
+    const float4 f = a.force[i], t = a.torque[i];
+    lv_second_half(lv, i, a.flags->step, a.seed, f.x, f.y, f.z, t.x, t.y, t.z, a.quat[i], v, p, bd, bdt);
+    a.vel[i] = v; a.angmom[i] = p; a.bd[i] = bd; a.bd_tq[i] = bdt;
+}
+
 }  // namespace oxd
 
 // =====================================================================================
@@ -966,12 +978,23 @@ StepArgs step_args(oxdna_ctx* c, bool with_cells, int parity) {
 // One step, number `step` (1-based): step1 (kick, drift, rotate, frames, binning, skin
 // test) -> forces (rows rebuilt in-kernel when the skin test fired; Langevin second
 // half-kick in the epilogue). Buffers alternate with the step's parity.
+// The second half-kick rides in the force kernel's epilogue while the force grid
+// leaves the GPU latency-bound (few waves per SIMD: a launch saved is time saved); on
+// larger systems its one-active-lane-per-particle epilogue would cost more issue slots
+// than a kernel with a thread per particle.
+bool fuse_second_half(const oxdna_ctx* c) {
+    const size_t waves = ((size_t)c->N * c->tpp + 63) / 64;
+    return waves <= 4096;  // ~4 waves per SIMD on 256 CUs
+}
+
 void enqueue_step(oxdna_ctx* c, uint32_t mask, uint64_t step) {
     const bool pairs = (mask & 0x3fu) != 0;
     const int parity = (int)(step & 1u);
+    const bool fuse = fuse_second_half(c);
     StepArgs s = step_args(c, pairs, parity);
     hipLaunchKernelGGL(k_step1, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, s);
-    launch_forces(c, mask, 0, false, false, pairs ? 2 : 0, parity, true);
+    launch_forces(c, mask, 0, false, false, pairs ? 2 : 0, parity, fuse);
+    if (!fuse) hipLaunchKernelGGL(k_step2, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, s);
 }
 
 constexpr int kGraphSteps = 16;

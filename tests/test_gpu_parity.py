@@ -346,3 +346,18 @@ def test_large_system_524k_nt(model):
     assert np.all(np.abs(e - ref["E"]) <= E_REL * np.abs(ref["E"]) + E_PER_NT * n)
     assert dF.max() <= F_REL * fmax + F_ABS
     assert dT.max() <= F_REL * tmax + F_ABS
+
+
+@pytest.mark.parametrize("model", [dna2, rna2])
+def test_thermalised_duplex_matches_oracle(model):
+    """The reference's CPU-runnable case (one 20-bp duplex, box 20, Langevin dt = 0.003):
+    after 1000 steps on the GPU, every term of the thermalised state against the
+    oracle (energies, forces, torques, per-particle energies, virials)."""
+    system = H.cfg1_system(0)
+    forces, _ = model.forces()
+    s = _langevin_sim(system, forces, seed=3)
+    s.run(1000)
+    snap = s.state.get_snapshot()
+    hot = systems.System(snap.particles.position, snap.particles.orientation, system.typeid, system.bonds,
+                         system.box, system.strand, system.bases, system.n3, system.n5, system.charge)
+    check_against_oracle(hot, model.forces()[0], f"cfg1 after 1000 steps {model.__name__.split('.')[-1]}")
