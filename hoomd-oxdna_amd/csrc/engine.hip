@@ -69,14 +69,16 @@ __global__ void k_zero_u32(uint32_t* p, size_t n) {
 // minimum-image distance, no contraction, bonded partners excluded), a lane prefix
 // sum places them in the LDS row, and a rank sort writes the row to LDS and HBM
 // (deterministic for any order of the cell members). Row entries are keys: the
-// neighbour index, with bit 31 set when the pair is beyond every cut-off but
-// Debye-Hueckel's even after the particles move by the skin; sorting by key puts the
-// near pairs first, so the lanes of later passes over the row all take the cheap
-// Debye-Hueckel-only path. Lanes of one particle are adjacent lanes of one wavefront;
-// every thread of the block reaches the barriers.
-constexpr uint32_t kFarBit = 0x80000000u;
+// neighbour index and, in bits 30-31, the terms the pair can reach while the particles
+// move less than the skin: 0 every term, 1 excluded volume and Debye-Hueckel only,
+// 2 Debye-Hueckel only. Sorting by key groups the classes, so a pass of the lanes over
+// the row mostly runs one class's path instead of the union of all of them. Lanes of one
+// particle are adjacent lanes of one wavefront; every thread of the block reaches the
+// barriers.
+constexpr uint32_t kClassShift = 30;
+constexpr uint32_t kIndexMask = (1u << kClassShift) - 1u;
 
-// 0: not a neighbour, 1: near neighbour, 2: far neighbour (Debye-Hueckel only).
+// 0: not a neighbour; else 1 + the pair's class (see above).
 template<int TPP>
 __device__ __forceinline__ int ref_pair(const NbrArgs& nb, const double4& pi, uint32_t i, uint32_t j, int b0, int b1) {
     if (j == i || (int)j == b0 || (int)j == b1) return 0;
@@ -92,7 +94,7 @@ __device__ __forceinline__ int ref_pair(const NbrArgs& nb, const double4& pi, ui
         r2 = r2 + dz * dz;
     }
     if (!(r2 < nb.rlist2)) return 0;
-    return r2 < nb.rnear2 ? 1 : 2;
+    return r2 < nb.rbb2 ? 1 : (r2 < nb.rnear2 ? 2 : 3);
 }
 
 template<int TPP>
@@ -114,7 +116,7 @@ __device__ __forceinline__ uint32_t scan_cells(const NbrArgs& nb, const double4&
             const uint32_t j = mem[k];
             const int cls = ref_pair<TPP>(nb, pi, i, j, b0, b1);
             if (cls != 0) {
-                if (write && base + n < nb.max_neigh) row[base + n] = cls == 2 ? (j | kFarBit) : j;
+                if (write && base + n < nb.max_neigh) row[base + n] = j | ((uint32_t)(cls - 1) << kClassShift);
                 ++n;
             }
         }
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
         const int ti = __float_as_int(f1.w);
         const int nt = P.ntypes;
         for (uint32_t k = lane; k < nn; k += TPP) {
-            const uint32_t j = row[k] & ~kFarBit;
+            const uint32_t j = row[k] & kIndexMask;
             const double4 pj = a.pos[j];
             const float4 g0 = a.fr0[j], g1 = a.fr1[j], g2 = a.fr2[j];
             Frame fj{mk(g0.x, g0.y, g0.z), mk(g1.x, g1.y, g1.z), mk(g2.x, g2.y, g2.z)};
@@ -856,11 +858,16 @@ ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild, i
     nb.count = c->cell_count.p + (size_t)parity * c->nc[0] * c->nc[1] * c->nc[2];
     nb.members = c->cell_members.p;
     nb.rlist2 = c->rlist * c->rlist;
-    // near: within the largest cut-off but Debye-Hueckel's, plus the skin
-    double rnear = 0.0;
-    for (int t = 0; t < 6; ++t)
-        if (t != T_DEBYE && ((mask >> t) & 1u)) rnear = std::max(rnear, c->rcut_max[t]);
+    // row classes (skin included): within reach of the base-base and coaxial terms; of
+    // excluded volume; of Debye-Hueckel only
+    double rbb = 0.0, rnear = 0.0;
+    for (int t = 0; t < 6; ++t) {
+        if (!((mask >> t) & 1u) || t == T_DEBYE) continue;
+        rnear = std::max(rnear, c->rcut_max[t]);
+        if (t != T_EXCVOL) rbb = std::max(rbb, c->rcut_max[t]);
+    }
     nb.rnear2 = ((mask >> T_DEBYE) & 1u) ? (rnear + c->buffer) * (rnear + c->buffer) : nb.rlist2;
+    nb.rbb2 = ((mask >> T_EXCVOL) & 1u) ? (rbb + c->buffer) * (rbb + c->buffer) : nb.rnear2;
     nb.max_neigh = c->max_neigh;
     nb.pos_last = c->pos_last.p;
     nb.flags = (rebuild != 0 && (mask & 0x3fu)) ? c->flags.p : nullptr;
@@ -1574,7 +1581,7 @@ int oxdna_get_nlist_pairs(oxdna_ctx* c, int32_t* pairs, size_t max_pairs, size_t
         for (uint32_t i = 0; i < n; ++i) {
             js.clear();
             for (uint32_t k = 0; k < nn[i]; ++k) {
-                uint32_t j = rows[(size_t)i * c->max_neigh + k] & 0x7fffffffu;  // drop the far-pair bit
+                uint32_t j = rows[(size_t)i * c->max_neigh + k] & kIndexMask;  // drop the class bits
                 if (j > i) js.push_back(j);
             }
             std::sort(js.begin(), js.end());  // rows hold near pairs first

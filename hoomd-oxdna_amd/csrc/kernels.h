@@ -168,7 +168,8 @@ struct NbrArgs {
     const uint32_t* count;
     const uint32_t* members;
     double rlist2;
-    double rnear2;         // closer than this (squared): a near pair; else Debye-Hueckel only
+    double rbb2;           // closer than this (squared): every term can reach the pair
+    double rnear2;         // closer than this: excluded volume + Debye-Hueckel; else Debye-Hueckel only
     uint32_t max_neigh;
     double4* pos_last;
     DevFlags* flags;       // NULL: never rebuild (rows supplied by the caller)
