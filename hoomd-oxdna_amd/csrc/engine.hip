@@ -349,32 +349,42 @@ __global__ void k_reduce_energy(uint32_t nblocks, const double* __restrict__ par
 }
 
 // ------------------------------------------------------------------ integrator
-// One particle of step1; returns its cell occupancy (0 without a cell list).
+// One particle of step1; returns its cell occupancy (0 without a cell list). The
+// translation, the skin test and the cell atomic come first, so the atomic's round trip
+// overlaps the long FP64 rotation chain.
 __device__ __forceinline__ uint32_t step1_particle(const StepArgs& a, uint32_t i) {
     const Langevin lv{a.dt, a.kT, a.gamma, a.gamma_r, a.mass, a.inertia};
-    double4 x = a.pos[i], v = a.vel[i], q = a.quat[i], p = a.angmom[i];
+    double4 x = a.pos[i], v = a.vel[i];
     // the cell this particle was binned into last step (other parity): clear it for
     // the step after this one (nobody reads or writes that buffer during this step)
     if (a.cells.count_prev) a.cells.count_prev[cell_index(x.x, x.y, x.z, a.cells.Linv, a.cells.nc)] = 0u;
-    const float4 f = a.force[i], t = a.torque[i];
-    lv_first_half(lv, a.L, a.Linv, f.x, f.y, f.z, t.x, t.y, t.z, a.bd[i], a.bd_tq[i], x, v, q, p);
-    a.pos[i] = x; a.vel[i] = v; a.quat[i] = q; a.angmom[i] = p;
+    const float4 f = a.force[i];
+    lv_first_half_translate(lv, a.L, a.Linv, f.x, f.y, f.z, a.bd[i], x, v);
+    a.pos[i] = x; a.vel[i] = v;
+    uint32_t cell = 0, slot = 0;
+    if (a.cells.count) {
+        cell = cell_index(x.x, x.y, x.z, a.cells.Linv, a.cells.nc);
+        slot = atomicAdd(&a.cells.count[cell], 1u);
+        double4 l = a.pos_last[i];
+        double dx = min_image(x.x - l.x, a.L[0], a.Linv[0]);
+        double dy = min_image(x.y - l.y, a.L[1], a.Linv[1]);
+        double dz = min_image(x.z - l.z, a.L[2], a.Linv[2]);
+        if (dx * dx + dy * dy + dz * dz > a.skin_half2) atomicOr(a.rebuild_set, 1u);
+    }
+    double4 q = a.quat[i], p = a.angmom[i];
+    const float4 t = a.torque[i];
+    lv_first_half_rotate(lv, t.x, t.y, t.z, a.bd_tq[i], q, p);
+    a.quat[i] = q; a.angmom[i] = p;
     float4 f0, f1, f2;
     quat_axes(q.x, q.y, q.z, q.w, f0, f1, f2);
     f0.w = a.charge ? (float)a.charge[i] : 1.f;
     f1.w = __int_as_float(type_of(x.w));
     f2.w = 0.f;
     a.fr0[i] = f0; a.fr1[i] = f1; a.fr2[i] = f2;
-    if (a.cells.count) {
-        const uint32_t occ = bin_particle(a.cells, i, x.x, x.y, x.z);
-        double4 l = a.pos_last[i];
-        double dx = min_image(x.x - l.x, a.L[0], a.Linv[0]);
-        double dy = min_image(x.y - l.y, a.L[1], a.Linv[1]);
-        double dz = min_image(x.z - l.z, a.L[2], a.Linv[2]);
-        if (dx * dx + dy * dy + dz * dz > a.skin_half2) atomicOr(a.rebuild_set, 1u);
-        return occ;
-    }
-    return 0;
+    if (!a.cells.count) return 0;
+    if (slot < a.cells.cap) a.cells.members[(size_t)cell * a.cells.cap + slot] = i;
+    else atomicOr(&a.flags->overflow, 1u);
+    return slot + 1;
 }
 
 // First half of the step (lv_first_half), then the new frames, cell binning, and the

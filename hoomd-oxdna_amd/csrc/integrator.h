@@ -87,10 +87,10 @@ struct Langevin {
 // rotation (axis 3: dt/2, 2: dt/2, 1: dt, 2: dt/2, 3: dt/2), renormalised q.
 // (f, t): conservative force and torque (lab frame); (bd, bdt): the Langevin force and
 // torque drawn in the previous second half. x.w (type bits) is left untouched.
-__device__ __forceinline__ void lv_first_half(const Langevin& lv, const double L[3], const double Linv[3],
-                                              float fx, float fy, float fz, float tx, float ty, float tz,
-                                              const double4& bd, const double4& bdt,
-                                              double4& x, double4& v, double4& q, double4& p) {
+// The translational part of lv_first_half (kick and drift).
+__device__ __forceinline__ void lv_first_half_translate(const Langevin& lv, const double L[3], const double Linv[3],
+                                                        float fx, float fy, float fz, const double4& bd,
+                                                        double4& x, double4& v) {
     const double dt = lv.dt;
     const double hm = 0.5 * dt / lv.mass;
     v.x += hm * (fx + bd.x); v.y += hm * (fy + bd.y); v.z += hm * (fz + bd.z);
@@ -98,6 +98,12 @@ __device__ __forceinline__ void lv_first_half(const Langevin& lv, const double L
     x.x -= L[0] * floor(x.x * Linv[0] + 0.5);
     x.y -= L[1] * floor(x.y * Linv[1] + 0.5);
     x.z -= L[2] * floor(x.z * Linv[2] + 0.5);
+}
+
+// The rotational part of lv_first_half (kick, NO_SQUISH rotation, renormalisation).
+__device__ __forceinline__ void lv_first_half_rotate(const Langevin& lv, float tx, float ty, float tz,
+                                                     const double4& bdt, double4& q, double4& p) {
+    const double dt = lv.dt;
     double tbx, tby, tbz;
     to_body(q, tx + bdt.x, ty + bdt.y, tz + bdt.z, tbx, tby, tbz);
     double4 kick = qmul(q, make_double4(0.0, tbx, tby, tbz));
@@ -110,6 +116,14 @@ __device__ __forceinline__ void lv_first_half(const Langevin& lv, const double L
     rotor(3, 0.5 * dt, inv4I, q, p);
     double nq = 1.0 / sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
     q.x *= nq; q.y *= nq; q.z *= nq; q.w *= nq;
+}
+
+__device__ __forceinline__ void lv_first_half(const Langevin& lv, const double L[3], const double Linv[3],
+                                              float fx, float fy, float fz, float tx, float ty, float tz,
+                                              const double4& bd, const double4& bdt,
+                                              double4& x, double4& v, double4& q, double4& p) {
+    lv_first_half_translate(lv, L, Linv, fx, fy, fz, bd, x, v);
+    lv_first_half_rotate(lv, tx, ty, tz, bdt, q, p);
 }
 
 // Second half for particle `i` at integration step `step` with random key `seed`:
