@@ -124,8 +124,95 @@ __device__ __forceinline__ uint32_t scan_cells(const NbrArgs& nb, const double4&
     return n;
 }
 
+// ------------------------------------------------------------------ integrator
+// First half of step t for particle i (lv_first_half) from its state after step t - 1
+// (x, v, q, p), that step's force and torque (f, tq) and Langevin force and torque
+// (bd, bdt); then the frames of step t, its cell binning and the skin test (against
+// plast, the position of the last row rebuild) that asks for a rebuild. xold: what
+// pos_out[i] holds (the position two steps back); charge: the particle's charge. The translation, the skin test and the cell atomic come
+// first, so the atomic's round trip overlaps the long FP64 rotation chain. Only a cell
+// overflow touches the flags (overflow bit, largest occupancy for the resize): device-
+// scope atomics on one address from every wavefront would serialise the launch.
+__device__ __forceinline__ void first_half_particle(const StepArgs& a, uint32_t i, double4 x, double4 v,
+                                                    double4 q, double4 p, const float4& f, const float4& tq,
+                                                    const double4& bd, const double4& bdt, const double4& xold,
+                                                    const double4& plast, float charge) {
+    if (a.cells.count_prev) {
+        // the cell this particle was binned into two steps back: clear it for step
+        // t + 1, which bins into that buffer
+        a.cells.count_prev[cell_index(xold.x, xold.y, xold.z, a.cells.Linv, a.cells.nc)] = 0u;
+    }
+    lv_first_half_translate(a.lv, a.L, a.Linv, f.x, f.y, f.z, bd, x, v);
+    a.pos_out[i] = x;
+    a.vel[i] = v;
+    uint32_t cell = 0, slot = 0;
+    if (a.cells.count) {
+        cell = cell_index(x.x, x.y, x.z, a.cells.Linv, a.cells.nc);
+        slot = atomicAdd(&a.cells.count[cell], 1u);
+        const double dx = min_image(x.x - plast.x, a.L[0], a.Linv[0]);
+        const double dy = min_image(x.y - plast.y, a.L[1], a.Linv[1]);
+        const double dz = min_image(x.z - plast.z, a.L[2], a.Linv[2]);
+        if (dx * dx + dy * dy + dz * dz > a.skin_half2) atomicOr(a.rebuild_set, 1u);
+    }
+    lv_first_half_rotate(a.lv, tq.x, tq.y, tq.z, bdt, q, p);
+    a.quat[i] = q;
+    a.angmom[i] = p;
+    float4 f0, f1, f2;
+    quat_axes(q.x, q.y, q.z, q.w, f0, f1, f2);
+    f0.w = charge;
+    f1.w = __int_as_float(type_of(x.w));
+    f2.w = 0.f;
+    a.fr0[i] = f0; a.fr1[i] = f1; a.fr2[i] = f2;
+    if (!a.cells.count) return;
+    if (slot < a.cells.cap) {
+        a.cells.members[(size_t)cell * a.cells.cap + slot] = i;
+    } else {
+        atomicOr(&a.flags->overflow, 1u);
+        atomicMax(&a.flags->max_cell, slot + 1);
+    }
+}
+
+// First half of step t from the stored state (the first step of a run segment).
+__global__ void k_step1(StepArgs a) {
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i == 0) *a.step_out = *a.step_in + 1ull;
+    if (i < a.N)
+        first_half_particle(a, i, a.pos_in[i], a.vel[i], a.quat[i], a.angmom[i], a.force[i], a.torque[i], a.bd[i],
+                            a.bd_tq[i], a.pos_out[i], a.pos_last[i], a.charge ? (float)a.charge[i] : 1.f);
+}
+
+// Second half of step t - 1 (lv_second_half) then first half of step t: the kernel
+// between two force launches on systems large enough to fill the GPU (smaller ones run
+// both halves in the force kernel's epilogue).
+__global__ void k_step21(StepArgs a) {
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i == 0) *a.step_out = *a.step_in + 1ull;
+    if (i < a.N) {
+        double4 v = a.vel[i], p = a.angmom[i], bd, bdt;
+        const double4 q = a.quat[i];
+        const float4 f = a.force[i], t = a.torque[i];
+        lv_second_half(a.lv, i, *a.step_in, a.seed, f.x, f.y, f.z, t.x, t.y, t.z, q, v, p, bd, bdt);
+        first_half_particle(a, i, a.pos_in[i], v, q, p, f, t, bd, bdt, a.pos_out[i], a.pos_last[i],
+                            a.charge ? (float)a.charge[i] : 1.f);
+    }
+}
+
+// Second half of step t - 1 alone (the last step of a run segment, large systems).
+__global__ void k_step2(StepArgs a) {
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= a.N) return;
+    double4 v = a.vel[i], p = a.angmom[i], bd, bdt;
+    const float4 f = a.force[i], t = a.torque[i];
+    lv_second_half(a.lv, i, *a.step_in, a.seed, f.x, f.y, f.z, t.x, t.y, t.z, a.quat[i], v, p, bd, bdt);
+    a.vel[i] = v; a.angmom[i] = p; a.bd[i] = bd; a.bd_tq[i] = bdt;
+}
+
 // ------------------------------------------------------------------ fused forces
-template<int TPP, bool WANT_E, bool WANT_V, bool COUNT>
+// FUSED: a run step whose epilogue also takes the particle through the Langevin second
+// half of step k and the first half of step k + 1 (a.integrate == 2); its loads are
+// issued before the pair loop, and the force and torque, consumed in the epilogue, are
+// not stored.
+template<int TPP, bool WANT_E, bool WANT_V, bool COUNT, bool FUSED = false>
 __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
     extern __shared__ uint32_t lds_rows[];
     const uint32_t tid = xcd_block() * blockDim.x + threadIdx.x;
@@ -141,6 +228,11 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
     const uint32_t* row = nullptr;
     double4 pi = make_double4(0, 0, 0, 0);
     if (active) pi = a.pos[i];
+    double4 e_v, e_p, e_q, e_xold, e_plast;
+    if (FUSED && active && lane == 0) {
+        const StepArgs& s = a.st;
+        e_v = s.vel[i]; e_p = s.angmom[i]; e_q = s.quat[i]; e_xold = s.pos_out[i]; e_plast = s.pos_last[i];
+    }
     if (rebuild) {
         // ---- neighbour row of i (uniform branch: every thread of the block is here)
         const uint32_t slot = threadIdx.x / TPP;
@@ -174,7 +266,10 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
                 a.n_neigh[i] = m;
                 nb.pos_last[i] = pi;
                 if (total > nb.max_neigh) atomicOr(&nb.flags->overflow, 2u);
-                atomicMax(&nb.flags->max_neigh, total);
+                // the largest count sizes the rows: wanted from a forced rebuild (which
+                // sizes them with headroom) and after an overflow; per-particle atomics
+                // on one address would serialise the ordinary rebuild steps
+                if (nb.force_rebuild || total > nb.max_neigh) atomicMax(&nb.flags->max_neigh, total);
                 if (i == 0) atomicAdd(&nb.flags->builds, 1ull);
             }
         }
@@ -188,6 +283,7 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
 
     V F = zero3(), T = zero3();
     float Eh = 0.f;
+    float charge_i = 1.f;
     float et[8];
     float vir[6];
 #pragma unroll
@@ -201,6 +297,7 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
         const float4 f0 = a.fr0[i], f1 = a.fr1[i], f2 = a.fr2[i];
         Frame fi{mk(f0.x, f0.y, f0.z), mk(f1.x, f1.y, f1.z), mk(f2.x, f2.y, f2.z)};
         const float qi = f0.w;
+        charge_i = qi;
         const int ti = __float_as_int(f1.w);
         const int nt = P.ntypes;
         for (uint32_t k = lane; k < nn; k += TPP) {
@@ -287,30 +384,43 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
     if (active && lane == 0) {
         float4 fo = make_float4(F.x, F.y, F.z, Eh);
         float4 to = make_float4(T.x, T.y, T.z, 0.f);
-        if (a.accumulate) {
-            float4 f = a.force[i], t = a.torque[i];
-            fo.x += f.x; fo.y += f.y; fo.z += f.z; fo.w += f.w;
-            to.x += t.x; to.y += t.y; to.z += t.z;
-        }
-        a.force[i] = fo;
-        a.torque[i] = to;
-        if (WANT_V) {
+        if (FUSED) {
+            // Langevin second half of step k with the force just computed, then the first
+            // half of step k + 1: both touch only particle i's own state, and the positions
+            // and frames of step k + 1 go to the other buffer of each pair, so no workgroup
+            // still reading step k sees them
+            const StepArgs& s = a.st;
+            double4 v = e_v, p = e_p, bd, bdt;
+            lv_second_half(s.lv, i, *s.step_in, s.seed, fo.x, fo.y, fo.z, to.x, to.y, to.z, e_q, v, p, bd, bdt);
+            first_half_particle(s, i, pi, v, e_q, p, fo, to, bd, bdt, e_xold, rebuild ? pi : e_plast, charge_i);
+        } else {
+            if (a.accumulate) {
+                float4 f = a.force[i], t = a.torque[i];
+                fo.x += f.x; fo.y += f.y; fo.z += f.z; fo.w += f.w;
+                to.x += t.x; to.y += t.y; to.z += t.z;
+            }
+            a.force[i] = fo;
+            a.torque[i] = to;
+            if (WANT_V) {
 #pragma unroll
-            for (int s = 0; s < 6; ++s) {
-                float v = vir[s];
-                if (a.accumulate) v += a.virial[s * a.vpitch + i];
-                a.virial[s * a.vpitch + i] = v;
+                for (int s = 0; s < 6; ++s) {
+                    float v = vir[s];
+                    if (a.accumulate) v += a.virial[s * a.vpitch + i];
+                    a.virial[s * a.vpitch + i] = v;
+                }
+            }
+            if (a.integrate) {
+                // the Langevin second half of step k alone (the last step of a run)
+                const StepArgs& s = a.st;
+                double4 v = s.vel[i], p = s.angmom[i], bd, bdt;
+                lv_second_half(s.lv, i, *s.step_in, s.seed, fo.x, fo.y, fo.z, to.x, to.y, to.z, s.quat[i], v, p,
+                               bd, bdt);
+                s.vel[i] = v; s.angmom[i] = p; s.bd[i] = bd; s.bd_tq[i] = bdt;
             }
         }
-        if (a.integrate) {
-            // second half of the Langevin step with the force just computed (touches
-            // only particle i's momenta, which no other particle's evaluation reads)
-            double4 v = a.vel[i], p = a.angmom[i], bd, bdt;
-            lv_second_half(a.lv, i, a.sflags->step, a.seed, fo.x, fo.y, fo.z, to.x, to.y, to.z, a.quat[i], v, p,
-                           bd, bdt);
-            a.vel[i] = v; a.angmom[i] = p; a.bd[i] = bd; a.bd_tq[i] = bdt;
-        }
     }
+    if (FUSED && tid == 0) *a.st.step_out = *a.st.step_in + 1ull;
+    if (a.rebuild_clear && tid == 0) *a.rebuild_clear = 0u;
     if (WANT_E) {
         // per-term totals: FP64 from here on (wave shuffle, then across waves in LDS)
         __shared__ double red[256 / 64][8];
@@ -346,72 +456,6 @@ __global__ void k_reduce_energy(uint32_t nblocks, const double* __restrict__ par
         if (threadIdx.x == 0) out[slot] = (accumulate ? out[slot] : 0.0) + s[0];
         __syncthreads();
     }
-}
-
-// ------------------------------------------------------------------ integrator
-// One particle of step1; returns its cell occupancy (0 without a cell list). The
-// translation, the skin test and the cell atomic come first, so the atomic's round trip
-// overlaps the long FP64 rotation chain.
-__device__ __forceinline__ uint32_t step1_particle(const StepArgs& a, uint32_t i) {
-    const Langevin lv{a.dt, a.kT, a.gamma, a.gamma_r, a.mass, a.inertia};
-    double4 x = a.pos[i], v = a.vel[i];
-    // the cell this particle was binned into last step (other parity): clear it for
-    // the step after this one (nobody reads or writes that buffer during this step)
-    if (a.cells.count_prev) a.cells.count_prev[cell_index(x.x, x.y, x.z, a.cells.Linv, a.cells.nc)] = 0u;
-    const float4 f = a.force[i];
-    lv_first_half_translate(lv, a.L, a.Linv, f.x, f.y, f.z, a.bd[i], x, v);
-    a.pos[i] = x; a.vel[i] = v;
-    uint32_t cell = 0, slot = 0;
-    if (a.cells.count) {
-        cell = cell_index(x.x, x.y, x.z, a.cells.Linv, a.cells.nc);
-        slot = atomicAdd(&a.cells.count[cell], 1u);
-        double4 l = a.pos_last[i];
-        double dx = min_image(x.x - l.x, a.L[0], a.Linv[0]);
-        double dy = min_image(x.y - l.y, a.L[1], a.Linv[1]);
-        double dz = min_image(x.z - l.z, a.L[2], a.Linv[2]);
-        if (dx * dx + dy * dy + dz * dz > a.skin_half2) atomicOr(a.rebuild_set, 1u);
-    }
-    double4 q = a.quat[i], p = a.angmom[i];
-    const float4 t = a.torque[i];
-    lv_first_half_rotate(lv, t.x, t.y, t.z, a.bd_tq[i], q, p);
-    a.quat[i] = q; a.angmom[i] = p;
-    float4 f0, f1, f2;
-    quat_axes(q.x, q.y, q.z, q.w, f0, f1, f2);
-    f0.w = a.charge ? (float)a.charge[i] : 1.f;
-    f1.w = __int_as_float(type_of(x.w));
-    f2.w = 0.f;
-    a.fr0[i] = f0; a.fr1[i] = f1; a.fr2[i] = f2;
-    if (!a.cells.count) return 0;
-    if (slot < a.cells.cap) a.cells.members[(size_t)cell * a.cells.cap + slot] = i;
-    else atomicOr(&a.flags->overflow, 1u);
-    return slot + 1;
-}
-
-// First half of the step (lv_first_half), then the new frames, cell binning, and the
-// skin test that requests a neighbour-list rebuild. The second half runs in the force
-// kernel's epilogue. Cell counts and rebuild requests alternate between two buffers by
-// step parity, so that one kernel never clears what another workgroup of it reads.
-__global__ void k_step1(StepArgs a) {
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        a.flags->step += 1ull;
-        *a.rebuild_clear = 0u;
-    }
-    uint32_t occ = 0;
-    if (i < a.N) occ = step1_particle(a, i);
-    if (a.cells.count) wave_atomic_max(&a.flags->max_cell, occ);
-}
-
-// Second half of the step (lv_second_half), as its own kernel (systems large enough
-// to fill the GPU; smaller ones run it in the force kernel's epilogue).
-__global__ void k_step2(StepArgs a) {
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= a.N) return;
-    const Langevin lv{a.dt, a.kT, a.gamma, a.gamma_r, a.mass, a.inertia};
-    double4 v = a.vel[i], p = a.angmom[i], bd, bdt;
-    const float4 f = a.force[i], t = a.torque[i];
-    lv_second_half(lv, i, a.flags->step, a.seed, f.x, f.y, f.z, t.x, t.y, t.z, a.quat[i], v, p, bd, bdt);
-    a.vel[i] = v; a.angmom[i] = p; a.bd[i] = bd; a.bd_tq[i] = bdt;
 }
 
 }  // namespace oxd
@@ -732,13 +776,13 @@ void upload_params(oxdna_ctx* c) {
 // ------------------------------------------------------------------ force launches
 // One templated launcher for every instantiation: TPP lanes per particle (8, 16, 32).
 void check_force_args(const ForceArgs& a);
-template<bool E, bool V, bool CNT>
+template<bool E, bool V, bool CNT, bool FUSED = false>
 void launch_k(int tpp, dim3 g, dim3 b, size_t lds, hipStream_t st, const ForceArgs& a) {
     check_force_args(a);
     switch (tpp) {
-    case 8: hipLaunchKernelGGL((k_forces<8, E, V, CNT>), g, b, lds, st, a); break;
-    case 16: hipLaunchKernelGGL((k_forces<16, E, V, CNT>), g, b, lds, st, a); break;
-    default: hipLaunchKernelGGL((k_forces<32, E, V, CNT>), g, b, lds, st, a); break;
+    case 8: hipLaunchKernelGGL((k_forces<8, E, V, CNT, FUSED>), g, b, lds, st, a); break;
+    case 16: hipLaunchKernelGGL((k_forces<16, E, V, CNT, FUSED>), g, b, lds, st, a); break;
+    default: hipLaunchKernelGGL((k_forces<32, E, V, CNT, FUSED>), g, b, lds, st, a); break;
     }
 }
 void check_force_args(const ForceArgs& a) {
@@ -749,12 +793,21 @@ void check_force_args(const ForceArgs& a) {
     if (a.nb.flags && (!a.nb.count || !a.nb.members || !a.nb.pos_last || !a.bpart))
         throw Fail("internal: row rebuild without a cell list");
     if (a.nb.rebuild_flag && !a.nb.flags) throw Fail("internal: rebuild request without a flags block");
-    if (a.integrate && (!a.sflags || !a.quat || !a.vel || !a.angmom || !a.bd || !a.bd_tq))
+    const StepArgs& t = a.st;
+    if (a.integrate && (!t.step_in || !t.quat || !t.vel || !t.angmom || !t.bd || !t.bd_tq))
         throw Fail("internal: fused Langevin half-kick without its arrays");
+    if (a.integrate == 2 && (!t.step_out || !t.pos_out || !t.pos_last || !t.force || !t.torque || !t.fr0 ||
+                             !t.fr1 || !t.fr2 || !t.flags || !t.rebuild_set || t.pos_out == a.pos))
+        throw Fail("internal: fused first half-kick without its arrays");
+    if (a.integrate == 2 && t.cells.count && (t.cells.count == a.nb.count || t.cells.members == a.nb.members))
+        throw Fail("internal: fused first half-kick bins into the cells the launch reads");
 }
 void launch_k_any(int tpp, bool e, bool v, dim3 g, dim3 b, size_t lds, hipStream_t st, const ForceArgs& a) {
     check_force_args(a);
-    if (e && v) launch_k<true, true, false>(tpp, g, b, lds, st, a);
+    if (a.integrate == 2) {
+        if (e || v || a.accumulate) throw Fail("internal: fused run step with energies, virial or accumulation");
+        launch_k<false, false, false, true>(tpp, g, b, lds, st, a);
+    } else if (e && v) launch_k<true, true, false>(tpp, g, b, lds, st, a);
     else if (e) launch_k<true, false, false>(tpp, g, b, lds, st, a);
     else if (v) launch_k<false, true, false>(tpp, g, b, lds, st, a);
     else launch_k<false, false, false>(tpp, g, b, lds, st, a);
@@ -765,6 +818,20 @@ constexpr size_t kMaxCells = size_t(1) << 24;
 
 // LDS of the in-kernel row rebuild: two rows (unsorted, sorted) per particle of a block.
 size_t lds_bytes(const oxdna_ctx* c, int tpp) { return (size_t)(kBlock / tpp) * 2 * c->max_neigh * sizeof(uint32_t); }
+
+size_t ncell_total(const oxdna_ctx* c) { return (size_t)c->nc[0] * c->nc[1] * c->nc[2]; }
+
+// Positions and frames come in two buffers of N: run step k's live in buffer k mod 2;
+// between runs the state is in buffer 0.
+double4* pos_buf(oxdna_ctx* c, uint64_t k) { return c->pos.p + (size_t)(k & 1) * c->N; }
+float4* fr_buf(DBuf<float4>& b, const oxdna_ctx* c, uint64_t k) { return b.p + (size_t)(k & 1) * c->N; }
+
+void zero_counts(oxdna_ctx* c) {
+    const size_t n = 3 * ncell_total(c);
+    if (n && c->cell_count.p)
+        hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for(n), 4096)), dim3(kBlock), 0, c->stream,
+                           c->cell_count.p, n);
+}
 
 void setup_cells(oxdna_ctx* c) {
     // neighbour radius: the largest cut-off of every attached nonbonded term + skin
@@ -782,10 +849,9 @@ void setup_cells(oxdna_ctx* c) {
         double per_cell = (double)c->N / (double)total;
         c->cell_cap = (uint32_t)std::max(16.0, std::ceil(per_cell * 4.0 + 8.0));
     }
-    c->cell_count.alloc(2 * total);  // two buffers, alternating by step parity
-    c->cell_members.alloc(total * c->cell_cap);
-    hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for(2 * total), 4096)), dim3(kBlock), 0, c->stream,
-                       c->cell_count.p, 2 * total);
+    c->cell_count.alloc(3 * total);        // counts of steps k mod 3
+    c->cell_members.alloc(2 * total * c->cell_cap);  // members of steps k mod 2
+    zero_counts(c);
     if (c->max_neigh == 0) c->max_neigh = 64;
     if (c->max_neigh > kMaxNeigh) throw Fail("neighbour rows would exceed " + std::to_string(kMaxNeigh) + " entries");
     c->nlist.alloc((size_t)c->N * c->max_neigh);
@@ -797,20 +863,19 @@ void setup_cells(oxdna_ctx* c) {
     if (lds_bytes(c, c->tpp) > kMaxLds) c->tpp = 32;
 }
 
-size_t ncell_total(const oxdna_ctx* c) { return (size_t)c->nc[0] * c->nc[1] * c->nc[2]; }
-
-// Cells of a step of parity `parity` (the counts buffer it bins into; the other one
-// is cleared as the particles leave it).
-CellArgs cell_args(oxdna_ctx* c, bool with_cells, int parity) {
+// Cells of run step t (t = 0: the state a run starts from, and single-point
+// evaluations): the counts it bins into (t mod 3; those of step t - 2, index
+// (t + 1) mod 3, are cleared as its particles leave them) and its members (t mod 2).
+CellArgs cell_args(oxdna_ctx* c, bool with_cells, uint64_t t) {
     CellArgs a;
     a.N = c->N;
-    a.pos = c->pos.p;
+    a.pos = pos_buf(c, t);
     for (int k = 0; k < 3; ++k) { a.Linv[k] = c->Linv[k]; a.nc[k] = c->nc[k]; }
     a.cap = c->cell_cap;
-    const size_t nct = (size_t)c->nc[0] * c->nc[1] * c->nc[2];
-    a.count = with_cells ? c->cell_count.p + (size_t)parity * nct : nullptr;
-    a.count_prev = with_cells ? c->cell_count.p + (size_t)(parity ^ 1) * nct : nullptr;
-    a.members = c->cell_members.p;
+    const size_t nct = ncell_total(c);
+    a.count = with_cells ? c->cell_count.p + (size_t)(t % 3) * nct : nullptr;
+    a.count_prev = with_cells ? c->cell_count.p + (size_t)((t + 1) % 3) * nct : nullptr;
+    a.members = c->cell_members.p + (size_t)(t & 1) * nct * c->cell_cap;
     a.flags = c->flags.p;
     return a;
 }
@@ -823,7 +888,7 @@ void read_flags(oxdna_ctx* c, DevFlags& f) {
 void clear_overflow(oxdna_ctx* c) {
     DevFlags f;
     read_flags(c, f);
-    f.overflow = 0; f.max_cell = 0; f.max_neigh = 0; f.rebuild[0] = f.rebuild[1] = 0;
+    f.overflow = 0; f.max_cell = 0; f.max_neigh = 0; f.rebuild[0] = f.rebuild[1] = f.rebuild[2] = 0;
     HIP_OK(hipMemcpyAsync(c->flags.p, &f, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
 }
 
@@ -837,14 +902,18 @@ void ensure_ready(oxdna_ctx* c) {
     upload_params(c);
 }
 
-// rebuild: 0 never (rows are current), 1 always, 2 when the skin test of the step of
-// parity `parity` asked for it; integrate: fuse the Langevin second half-kick.
-ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild, int parity = 0,
-                     bool integrate = false) {
+StepArgs step_args(oxdna_ctx* c, bool with_cells, uint64_t t);
+
+// Forces of run step k (k = 0: the stored state). rebuild: 0 never (rows are current),
+// 1 always, 2 when step k's skin test asked for it (run steps, which also clear step
+// k + 2's request); integrate: 0 none, 1 the Langevin second half of step k, 2 that and
+// the first half of step k + 1, in the epilogue.
+ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild, uint64_t k = 0,
+                     int integrate = 0) {
     ForceArgs a{};
     a.N = c->N;
-    a.pos = c->pos.p;
-    a.fr0 = c->fr0.p; a.fr1 = c->fr1.p; a.fr2 = c->fr2.p;
+    a.pos = pos_buf(c, k);
+    a.fr0 = fr_buf(c->fr0, c, k); a.fr1 = fr_buf(c->fr1, c, k); a.fr2 = fr_buf(c->fr2, c, k);
     a.n_neigh = c->n_neigh.p;
     a.nlist = c->nlist.p;
     a.head = c->head.p;
@@ -862,11 +931,12 @@ ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild, i
     a.split = nullptr;
     a.counts = nullptr;
     NbrArgs& nb = a.nb;
-    nb.pos = c->pos.p;
-    for (int k = 0; k < 3; ++k) { nb.L[k] = c->L[k]; nb.Linv[k] = c->Linv[k]; nb.nc[k] = c->nc[k]; }
+    nb.pos = a.pos;
+    for (int d = 0; d < 3; ++d) { nb.L[d] = c->L[d]; nb.Linv[d] = c->Linv[d]; nb.nc[d] = c->nc[d]; }
     nb.cap = c->cell_cap;
-    nb.count = c->cell_count.p + (size_t)parity * c->nc[0] * c->nc[1] * c->nc[2];
-    nb.members = c->cell_members.p;
+    const CellArgs cells = cell_args(c, true, k);
+    nb.count = cells.count;
+    nb.members = cells.members;
     nb.rlist2 = c->rlist * c->rlist;
     // row classes (skin included): within reach of the base-base and coaxial terms; of
     // excluded volume; of Debye-Hueckel only
@@ -881,25 +951,20 @@ ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild, i
     nb.max_neigh = c->max_neigh;
     nb.pos_last = c->pos_last.p;
     nb.flags = (rebuild != 0 && (mask & 0x3fu)) ? c->flags.p : nullptr;
-    nb.rebuild_flag = (rebuild == 2 && (mask & 0x3fu)) ? &c->flags.p->rebuild[parity] : nullptr;
-    nb.force_rebuild = (rebuild == 1 && (mask & 0x3fu)) ? 1 : 0;
-    a.integrate = integrate ? 1 : 0;
-    a.lv = Langevin{c->dt, c->kT, c->gamma, c->gamma_r, 1.0, 1.0};
-    a.seed = c->seed;
-    a.sflags = c->flags.p;
-    a.quat = c->quat.p;
-    a.vel = c->vel.p;
-    a.angmom = c->angmom.p;
-    a.bd = c->bd.p;
-    a.bd_tq = c->bd_tq.p;
+    const bool pairs = (mask & 0x3fu) != 0;
+    nb.rebuild_flag = (rebuild == 2 && pairs) ? &c->flags.p->rebuild[k % 3] : nullptr;
+    nb.force_rebuild = (rebuild == 1 && pairs) ? 1 : 0;
+    a.rebuild_clear = (rebuild == 2 && pairs) ? &c->flags.p->rebuild[(k + 2) % 3] : nullptr;
+    a.integrate = integrate;
+    if (integrate) a.st = step_args(c, pairs, k + 1);
     return a;
 }
 
 uint32_t force_blocks(const oxdna_ctx* c, int tpp) { return blocks_for((size_t)c->N * tpp); }
 
 void launch_forces(oxdna_ctx* c, uint32_t mask, int accumulate, bool want_e, bool want_v, int rebuild,
-                   int parity = 0, bool integrate = false) {
-    ForceArgs a = force_args(c, mask, accumulate, rebuild, parity, integrate);
+                   uint64_t k = 0, int integrate = 0) {
+    ForceArgs a = force_args(c, mask, accumulate, rebuild, k, integrate);
     dim3 grid(force_blocks(c, c->tpp)), block(kBlock);
     size_t lds = a.nb.flags ? lds_bytes(c, c->tpp) : 0;
     launch_k_any(c->tpp, want_e, want_v, grid, block, lds, c->stream, a);
@@ -916,10 +981,7 @@ void compute_rebuild(oxdna_ctx* c, uint32_t m, int fused) {
     for (int attempt = 0; attempt < 8; ++attempt) {
         if (pairs) {
             clear_overflow(c);
-            // a run leaves its last step's counts in one buffer: start from empty cells
-            size_t nct2 = 2 * ncell_total(c);
-            hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for(nct2), 4096)), dim3(kBlock), 0,
-                               c->stream, c->cell_count.p, nct2);
+            zero_counts(c);  // start from empty cells
             hipLaunchKernelGGL(k_cell_assign, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream,
                                cell_args(c, true, 0));
         }
@@ -938,9 +1000,7 @@ void compute_rebuild(oxdna_ctx* c, uint32_t m, int fused) {
             c->nlist_valid = true;
             return;
         }
-        size_t nct = 2 * ncell_total(c);  // both parities start the next step empty
-        hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for(nct), 4096)), dim3(kBlock), 0, c->stream,
-                           c->cell_count.p, nct);
+        zero_counts(c);  // every buffer starts the next run empty
         DevFlags f;
         read_flags(c, f);
         bool grow = false;
@@ -973,56 +1033,68 @@ void compute(oxdna_ctx* c, uint32_t mask, int fused) {
     c->forces_mask = m;
 }
 
-StepArgs step_args(oxdna_ctx* c, bool with_cells, int parity) {
+// First half of run step t (t >= 1): from buffer (t - 1) mod 2 to buffer t mod 2.
+StepArgs step_args(oxdna_ctx* c, bool with_cells, uint64_t t) {
     StepArgs a;
     a.N = c->N;
     a.bd = c->bd.p; a.bd_tq = c->bd_tq.p;
-    a.pos = c->pos.p; a.vel = c->vel.p; a.quat = c->quat.p; a.angmom = c->angmom.p;
+    a.pos_in = pos_buf(c, t - 1);
+    a.pos_out = pos_buf(c, t);
+    a.vel = c->vel.p; a.quat = c->quat.p; a.angmom = c->angmom.p;
     a.force = c->force.p; a.torque = c->torque.p; a.charge = c->charge.p;
-    a.fr0 = c->fr0.p; a.fr1 = c->fr1.p; a.fr2 = c->fr2.p;
+    a.fr0 = fr_buf(c->fr0, c, t); a.fr1 = fr_buf(c->fr1, c, t); a.fr2 = fr_buf(c->fr2, c, t);
     a.pos_last = c->pos_last.p;
-    a.dt = c->dt; a.kT = c->kT; a.gamma = c->gamma; a.gamma_r = c->gamma_r; a.mass = 1.0; a.inertia = 1.0;
+    a.lv = make_langevin(c->dt, c->kT, c->gamma, c->gamma_r, 1.0, 1.0);
     for (int k = 0; k < 3; ++k) { a.L[k] = c->L[k]; a.Linv[k] = c->Linv[k]; }
     a.skin_half2 = 0.25 * c->buffer * c->buffer;
     a.seed = c->seed;
     a.flags = c->flags.p;
-    a.rebuild_set = &c->flags.p->rebuild[parity];
-    a.rebuild_clear = &c->flags.p->rebuild[parity ^ 1];
-    a.cells = cell_args(c, with_cells, parity);
+    a.rebuild_set = &c->flags.p->rebuild[t % 3];
+    a.step_in = &c->flags.p->stepc[(t - 1) & 1];
+    a.step_out = &c->flags.p->stepc[t & 1];
+    a.cells = cell_args(c, with_cells, t);
     return a;
 }
 
-// One step, number `step` (1-based): step1 (kick, drift, rotate, frames, binning, skin
-// test) -> forces (rows rebuilt in-kernel when the skin test fired; Langevin second
-// half-kick in the epilogue). Buffers alternate with the step's parity.
-// The second half-kick rides in the force kernel's epilogue while the force grid
-// leaves the GPU latency-bound (few waves per SIMD: a launch saved is time saved); on
-// larger systems its one-active-lane-per-particle epilogue would cost more issue slots
-// than a kernel with a thread per particle.
+// The Langevin second half-kick rides in the force kernel's epilogue (with the next
+// step's first half) while the force grid leaves the GPU latency-bound (few waves per
+// SIMD: a launch saved is time saved); on larger systems its one-active-lane-per-
+// particle epilogue would cost more issue slots than a kernel with a thread per particle.
 bool fuse_second_half(const oxdna_ctx* c) {
     const size_t waves = ((size_t)c->N * c->tpp + 63) / 64;
-    return waves <= 4096;  // ~4 waves per SIMD on 256 CUs
+    return waves <= 3072;  // resident at once: 3 waves per SIMD (the fused kernel's occupancy) on 256 CUs
 }
 
-void enqueue_step(oxdna_ctx* c, uint32_t mask, uint64_t step) {
+// A chunk of a run is: the first half of step 1 (k_step1), inner steps 1 ... n - 1 (the
+// forces of step k, its second half and the first half of step k + 1: one fused force
+// launch, or a force launch and k_step21 on large systems), and a last step n (forces
+// and second half only), which leaves the state complete.
+void enqueue_inner(oxdna_ctx* c, uint32_t mask, uint64_t k0, uint32_t len) {
     const bool pairs = (mask & 0x3fu) != 0;
-    const int parity = (int)(step & 1u);
     const bool fuse = fuse_second_half(c);
-    StepArgs s = step_args(c, pairs, parity);
-    hipLaunchKernelGGL(k_step1, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, s);
-    launch_forces(c, mask, 0, false, false, pairs ? 2 : 0, parity, fuse);
-    if (!fuse) hipLaunchKernelGGL(k_step2, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, s);
+    for (uint64_t k = k0 + 1; k <= k0 + len; ++k) {
+        launch_forces(c, mask, 0, false, false, pairs ? 2 : 0, k, fuse ? 2 : 0);
+        if (!fuse)
+            hipLaunchKernelGGL(k_step21, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, step_args(c, pairs, k + 1));
+    }
 }
 
-constexpr int kGraphSteps = 16;
+void enqueue_last(oxdna_ctx* c, uint32_t mask, uint64_t k) {
+    const bool pairs = (mask & 0x3fu) != 0;
+    const bool fuse = fuse_second_half(c);
+    launch_forces(c, mask, 0, false, false, pairs ? 2 : 0, k, fuse ? 1 : 0);
+    if (!fuse) hipLaunchKernelGGL(k_step2, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, step_args(c, pairs, k + 1));
+}
+
+// Inner steps per graph: a multiple of 6, so that its buffer indices (step mod 2 and
+// mod 3) are the same at every replay.
+constexpr uint32_t kGraphSteps = 96;
 
 void build_graph(oxdna_ctx* c, uint32_t mask) {
     c->drop_graph();
     hipGraph_t g;
-    // replayed only when an even number of steps is done: step k of the graph is step
-    // number done + 1 + k, of parity (1 + k) & 1
     HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int s = 0; s < kGraphSteps; ++s) enqueue_step(c, mask, (uint64_t)s + 1);
+    enqueue_inner(c, mask, 0, kGraphSteps);
     HIP_OK(hipStreamEndCapture(c->stream, &g));
     HIP_OK(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
     HIP_OK(hipGraphDestroy(g));
@@ -1098,53 +1170,68 @@ void restore_state(oxdna_ctx* c, Saved& s) {
     cp(c->bd, s.bd); cp(c->bd_tq, s.bd_tq);
 }
 
+// Runs `nsteps` Langevin steps in chunks; a chunk whose cells or rows overflowed is
+// run again from its saved start with larger capacities.
 void run(oxdna_ctx* c, uint32_t nsteps, uint32_t mask, int use_graph) {
     if (!c->have_langevin) throw Fail("no integrator: call oxdna_set_langevin first");
     ensure_ready(c);
     uint32_t m = effective_mask(c, mask);
     if (c->forces_mask != m || !c->nlist_valid) compute(c, m, 1);
     autotune(c, m);
-    const uint32_t chunk_max = 4096;
+    const uint32_t chunk_max = 42 * kGraphSteps + 1;
     uint32_t done = 0;
     Saved saved;
     while (done < nsteps) {
-        uint32_t chunk = std::min(chunk_max, nsteps - done);
+        const uint32_t chunk = std::min(chunk_max, nsteps - done);
         save_state(c, saved);
         DevFlags f0;
         read_flags(c, f0);
-        unsigned long long step0 = f0.step;
+        const unsigned long long step0 = f0.step;
         for (int attempt = 0;; ++attempt) {
-            uint32_t left = chunk;
-            uint64_t step = step0;  // steps done so far
-            if (use_graph) {
+            // the chunk's steps are numbered 1 ... chunk from its starting state (step 0):
+            // random-stream counter, empty cells, no rebuild requests
+            f0.stepc[0] = step0;
+            f0.rebuild[0] = f0.rebuild[1] = f0.rebuild[2] = 0;
+            HIP_OK(hipMemcpyAsync(c->flags.p, &f0, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
+            zero_counts(c);
+            const bool pairs = (m & 0x3fu) != 0;
+            hipLaunchKernelGGL(k_step1, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, step_args(c, pairs, 1));
+            uint32_t k = 0;  // inner steps enqueued
+            if (use_graph && chunk - 1 >= kGraphSteps) {
                 if (!c->graph || c->graph_mask != m) build_graph(c, m);
-                if ((step & 1u) && left > 0) {  // the graph starts at an even count
-                    enqueue_step(c, m, ++step);
-                    --left;
-                }
-                while (left >= (uint32_t)c->graph_steps) {
-                    HIP_OK(hipGraphLaunch(c->graph, c->stream));
-                    left -= c->graph_steps;
-                    step += c->graph_steps;
-                }
+                for (; k + kGraphSteps <= chunk - 1; k += kGraphSteps) HIP_OK(hipGraphLaunch(c->graph, c->stream));
             }
-            while (left > 0) {
-                enqueue_step(c, m, ++step);
-                --left;
-            }
+            // the rest without the graph: the same launches, so both paths are bit-identical
+            enqueue_inner(c, m, k, chunk - 1 - k);
+            enqueue_last(c, m, chunk);
             DevFlags f;
             read_flags(c, f);
-            if (f.overflow == 0) break;
+            if (f.overflow == 0) {
+                // the chunk's state is in buffer chunk mod 2: bring it to buffer 0
+                if (chunk & 1u) {
+                    const size_t n = c->N;
+                    HIP_OK(hipMemcpyAsync(c->pos.p, c->pos.p + n, sizeof(double4) * n, hipMemcpyDeviceToDevice,
+                                          c->stream));
+                    for (DBuf<float4>* b : {&c->fr0, &c->fr1, &c->fr2})
+                        HIP_OK(hipMemcpyAsync(b->p, b->p + n, sizeof(float4) * n, hipMemcpyDeviceToDevice,
+                                              c->stream));
+                }
+                f.step = step0 + chunk;
+                HIP_OK(hipMemcpyAsync(c->flags.p, &f, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
+                HIP_OK(hipStreamSynchronize(c->stream));
+                break;
+            }
             if (attempt >= 4) throw Fail("neighbour list kept overflowing during the run");
             // grow, restore the chunk's starting state and run it again
             restore_state(c, saved);
             c->max_neigh = std::max(c->max_neigh, (uint32_t)std::ceil(f.max_neigh * 1.5) + 8);
             c->cell_cap = std::max(c->cell_cap, (uint32_t)std::ceil(f.max_cell * 1.5) + 4);
-            f.overflow = 0; f.step = step0; f.rebuild[0] = f.rebuild[1] = 0;
+            f.overflow = 0; f.step = step0;
             HIP_OK(hipMemcpyAsync(c->flags.p, &f, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
             c->drop_graph();
             setup_cells(c);
             compute(c, m, 1);
+            read_flags(c, f0);
         }
         done += chunk;
     }
@@ -1197,7 +1284,7 @@ EnsArgs ens_args(oxdna_ctx* c, uint32_t mask, uint32_t nsteps, uint32_t nevals, 
     a.force = c->force.p; a.torque = c->torque.p;
     a.rep_energy = c->rep_energy.p;
     a.bias = c->have_bias ? c->rep_bias.p : nullptr;
-    a.lv = Langevin{c->dt, c->kT, c->gamma, c->gamma_r, 1.0, 1.0};
+    a.lv = make_langevin(c->dt, c->kT, c->gamma, c->gamma_r, 1.0, 1.0);
     a.seed = c->seed;
     a.step0 = c->host_step;
     a.nsteps = nsteps;
@@ -1402,12 +1489,14 @@ int oxdna_set_state(oxdna_ctx* c, unsigned int n, const double* pos, const doubl
             c->n_rep = 0;  // the replica layout no longer fits
             c->have_bias = false;
         }
-        c->pos.alloc(n); c->quat.alloc(n); c->vel.alloc(n); c->angmom.alloc(n); c->pos_last.alloc(n);
+        c->pos.alloc(2 * (size_t)n); c->quat.alloc(n); c->vel.alloc(n); c->angmom.alloc(n); c->pos_last.alloc(n);
         c->bd.alloc(n); c->bd_tq.alloc(n);
-        c->charge.alloc(n); c->fr0.alloc(n); c->fr1.alloc(n); c->fr2.alloc(n);
+        c->charge.alloc(n); c->fr0.alloc(2 * (size_t)n); c->fr1.alloc(2 * (size_t)n); c->fr2.alloc(2 * (size_t)n);
         c->force.alloc(n); c->torque.alloc(n); c->virial.alloc(6 * (size_t)n); c->n_neigh.alloc(n);
         c->epart.alloc((size_t)blocks_for((size_t)n * kTPPMax) * 8);
         HIP_OK(hipMemcpyAsync(c->pos.p, p.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
+        // the second buffer too: a run reads its stale positions (to clear cells)
+        HIP_OK(hipMemcpyAsync(c->pos.p + n, p.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
         HIP_OK(hipMemcpyAsync(c->pos_last.p, p.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
         HIP_OK(hipMemcpyAsync(c->quat.p, q.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
         HIP_OK(hipMemcpyAsync(c->charge.p, ch.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));

@@ -33,21 +33,39 @@ __device__ __forceinline__ double4 qmul(double4 a, double4 b) {
                         a.x * b.w + a.y * b.z - a.z * b.y + a.w * b.x);
 }
 __device__ __forceinline__ double4 qconj(double4 a) { return make_double4(a.x, -a.y, -a.z, -a.w); }
-// rotate the lab vector v into the body frame: conj(q) v q
-__device__ __forceinline__ void to_body(double4 q, double vx, double vy, double vz, double& bx, double& by, double& bz) {
-    double4 t = qmul(qmul(qconj(q), make_double4(0.0, vx, vy, vz)), q);
-    bx = t.y; by = t.z; bz = t.w;
+// Rotation matrix of a unit quaternion: lab = R body (columns: the body axes a1, a2, a3).
+struct Rot {
+    double m[9];
+};
+__device__ __forceinline__ Rot rot_of(const double4& q) {
+    const double w = q.x, x = q.y, y = q.z, z = q.w;
+    Rot r;
+    r.m[0] = 1.0 - 2.0 * (y * y + z * z); r.m[1] = 2.0 * (x * y - z * w);       r.m[2] = 2.0 * (x * z + y * w);
+    r.m[3] = 2.0 * (x * y + z * w);       r.m[4] = 1.0 - 2.0 * (x * x + z * z); r.m[5] = 2.0 * (y * z - x * w);
+    r.m[6] = 2.0 * (x * z - y * w);       r.m[7] = 2.0 * (y * z + x * w);       r.m[8] = 1.0 - 2.0 * (x * x + y * y);
+    return r;
 }
-__device__ __forceinline__ void to_lab(double4 q, double vx, double vy, double vz, double& bx, double& by, double& bz) {
-    double4 t = qmul(qmul(q, make_double4(0.0, vx, vy, vz)), qconj(q));
-    bx = t.y; by = t.z; bz = t.w;
+// lab vector v into the body frame (R^T v = conj(q) v q for a unit q)
+__device__ __forceinline__ void to_body(const Rot& r, double vx, double vy, double vz, double& bx, double& by,
+                                        double& bz) {
+    bx = r.m[0] * vx + r.m[3] * vy + r.m[6] * vz;
+    by = r.m[1] * vx + r.m[4] * vy + r.m[7] * vz;
+    bz = r.m[2] * vx + r.m[5] * vy + r.m[8] * vz;
 }
-// cos and sin of x: Taylor series through x^7 / x^6 for |x| <= 2^-10 (the first
-// dropped terms, x^9/9! and x^8/8!, are below 2^-100 relative: exact in FP64), through
-// x^13 for |x| <= 1/4 (truncation below 2^-60), the library functions otherwise. A
-// Langevin rotor turns by |omega| dt / 2 ~ 1e-3 rad, so the short series is the rule.
+// body vector v into the lab frame (R v = q v conj(q))
+__device__ __forceinline__ void to_lab(const Rot& r, double vx, double vy, double vz, double& lx, double& ly,
+                                       double& lz) {
+    lx = r.m[0] * vx + r.m[1] * vy + r.m[2] * vz;
+    ly = r.m[3] * vx + r.m[4] * vy + r.m[5] * vz;
+    lz = r.m[6] * vx + r.m[7] * vy + r.m[8] * vz;
+}
+// cos and sin of x: Taylor series through x^7 / x^6 for |x| <= 2^-5 (the first dropped
+// terms, x^9/9! and x^8/8!, are below 2.5e-18 and 2.3e-17 relative, under half an FP64
+// ulp), through x^13 for |x| <= 1/4 (truncation below 2^-60), the library functions
+// otherwise. A Langevin rotor turns by |omega| dt / 2 ~ 1e-3 rad: the short series is
+// the rule, and a wavefront rarely runs a second branch.
 __device__ __forceinline__ void sincos_small(double x, double& s, double& c) {
-    if (fabs(x) <= 0.0009765625) {
+    if (fabs(x) <= 0.03125) {
         double x2 = x * x;
         s = x * (1.0 + x2 * (-1.0 / 6 + x2 * (1.0 / 120 + x2 * (-1.0 / 5040))));
         c = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24 + x2 * (-1.0 / 720)));
@@ -80,7 +98,35 @@ __device__ __forceinline__ void rotor(int k, double tau, double inv4I, double4& 
 
 struct Langevin {
     double dt, kT, gamma, gamma_r, mass, inertia;
+    // derived constants (make_langevin), so the per-particle chain holds no division
+    // or square root of its own
+    double hm;         // dt / (2 m)
+    double ct;         // sqrt(6 gamma kT / dt): half-width of the uniform force noise
+    double cr;         // sqrt(2 gamma_r kT / dt): deviation of the Gaussian torque noise
+    double gr_half_I;  // gamma_r / (2 I): drag on a component of q* (x) p = (0, 2 L_body)
+    double inv4I;      // 1 / (4 I)
 };
+
+__host__ __device__ inline Langevin make_langevin(double dt, double kT, double gamma, double gamma_r, double mass,
+                                                  double inertia) {
+    Langevin lv;
+    lv.dt = dt; lv.kT = kT; lv.gamma = gamma; lv.gamma_r = gamma_r; lv.mass = mass; lv.inertia = inertia;
+    lv.hm = 0.5 * dt / mass;
+    lv.ct = sqrt(6.0 * gamma * kT / dt);
+    lv.cr = sqrt(2.0 * gamma_r * kT / dt);
+    lv.gr_half_I = 0.5 * gamma_r / inertia;
+    lv.inv4I = 0.25 / inertia;
+    return lv;
+}
+
+// 1 / sqrt(s) for the squared norm of a quaternion that left the rotor chain: within
+// 2^-20 of 1 (always, but for a caller's unnormalised input) by 1 - d/2 + 3 d^2/8,
+// d = s - 1, whose truncation (5/16 |d|^3 < 2^-61) is below half an FP64 ulp.
+__device__ __forceinline__ double rsqrt_near1(double s) {
+    const double d = s - 1.0;
+    if (fabs(d) <= 0x1p-20) return 1.0 + d * (-0.5 + 0.375 * d);
+    return 1.0 / sqrt(s);
+}
 
 // First half of a step for one particle: v += dt/2 (F + F_L)/m, x += dt v (wrapped
 // into the box centred on the origin), p += dt q (x) (tau + tau_L)_body, NO_SQUISH
@@ -92,7 +138,7 @@ __device__ __forceinline__ void lv_first_half_translate(const Langevin& lv, cons
                                                         float fx, float fy, float fz, const double4& bd,
                                                         double4& x, double4& v) {
     const double dt = lv.dt;
-    const double hm = 0.5 * dt / lv.mass;
+    const double hm = lv.hm;
     v.x += hm * (fx + bd.x); v.y += hm * (fy + bd.y); v.z += hm * (fz + bd.z);
     x.x += dt * v.x; x.y += dt * v.y; x.z += dt * v.z;
     x.x -= L[0] * floor(x.x * Linv[0] + 0.5);
@@ -105,16 +151,16 @@ __device__ __forceinline__ void lv_first_half_rotate(const Langevin& lv, float t
                                                      const double4& bdt, double4& q, double4& p) {
     const double dt = lv.dt;
     double tbx, tby, tbz;
-    to_body(q, tx + bdt.x, ty + bdt.y, tz + bdt.z, tbx, tby, tbz);
+    to_body(rot_of(q), tx + bdt.x, ty + bdt.y, tz + bdt.z, tbx, tby, tbz);
     double4 kick = qmul(q, make_double4(0.0, tbx, tby, tbz));
     p.x += dt * kick.x; p.y += dt * kick.y; p.z += dt * kick.z; p.w += dt * kick.w;
-    const double inv4I = 0.25 / lv.inertia;
+    const double inv4I = lv.inv4I;
     rotor(3, 0.5 * dt, inv4I, q, p);
     rotor(2, 0.5 * dt, inv4I, q, p);
     rotor(1, dt, inv4I, q, p);
     rotor(2, 0.5 * dt, inv4I, q, p);
     rotor(3, 0.5 * dt, inv4I, q, p);
-    double nq = 1.0 / sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    const double nq = rsqrt_near1(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
     q.x *= nq; q.y *= nq; q.z *= nq; q.w *= nq;
 }
 
@@ -140,29 +186,29 @@ __device__ __forceinline__ void lv_second_half(const Langevin& lv, uint32_t i, u
     philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
     uint32_t d[4] = {i, (uint32_t)step, (uint32_t)(step >> 32), 1u};
     philox(d, (uint32_t)seed, (uint32_t)(seed >> 32));
-    const double ct = sqrt(6.0 * lv.gamma * lv.kT / dt);
+    const double ct = lv.ct;
     double bx = -lv.gamma * v.x + ct * (2.0 * u01f(c[0]) - 1.0);
     double by = -lv.gamma * v.y + ct * (2.0 * u01f(c[1]) - 1.0);
     double bz = -lv.gamma * v.z + ct * (2.0 * u01f(c[2]) - 1.0);
-    const double hm = 0.5 * dt / lv.mass;
+    const double hm = lv.hm;
     v.x += hm * (fx + bx); v.y += hm * (fy + by); v.z += hm * (fz + bz);
     bd = make_double4(bx, by, bz, 0.0);
+    const Rot R = rot_of(q);
     double tbx, tby, tbz;
-    to_body(q, tx, ty, tz, tbx, tby, tbz);
+    to_body(R, tx, ty, tz, tbx, tby, tbz);
     double4 Lb = qmul(qconj(q), p);  // = 2 (0, L_body)
-    const double I = lv.inertia;
-    const double cr = sqrt(2.0 * lv.gamma_r * lv.kT / dt);
+    const double cr = lv.cr, gr = lv.gr_half_I;
     // Box-Muller in FP32: the noise needs no FP64 accuracy
     float r1 = sqrtf(-2.f * __logf(u01f(c[3]))), r2 = sqrtf(-2.f * __logf(u01f(d[1])));
     float s0, c0, s2, c2;
     __sincosf(6.2831853f * u01f(d[0]), &s0, &c0);
     __sincosf(6.2831853f * u01f(d[2]), &s2, &c2);
-    double rbx = cr * (double)(r1 * c0) - lv.gamma_r * (0.5 * Lb.y) / I;
-    double rby = cr * (double)(r1 * s0) - lv.gamma_r * (0.5 * Lb.z) / I;
-    double rbz = cr * (double)(r2 * c2) - lv.gamma_r * (0.5 * Lb.w) / I;
+    double rbx = cr * (double)(r1 * c0) - gr * Lb.y;
+    double rby = cr * (double)(r1 * s0) - gr * Lb.z;
+    double rbz = cr * (double)(r2 * c2) - gr * Lb.w;
     tbx += rbx; tby += rby; tbz += rbz;
     double lx, ly, lz;
-    to_lab(q, rbx, rby, rbz, lx, ly, lz);
+    to_lab(R, rbx, rby, rbz, lx, ly, lz);
     bdt = make_double4(lx, ly, lz, 0.0);
     double4 kick = qmul(q, make_double4(0.0, tbx, tby, tbz));
     p.x += dt * kick.x; p.y += dt * kick.y; p.z += dt * kick.z; p.w += dt * kick.w;

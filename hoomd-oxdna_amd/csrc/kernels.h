@@ -14,14 +14,21 @@ namespace oxd {
 enum { T_EXCVOL = 0, T_HBOND = 1, T_CROSS = 2, T_COAX = 3, T_DEBYE = 4, T_HBONDCROSS = 5, T_BONDED = 6 };
 enum { S_FENE = 0, S_BEXC = 1, S_STACK = 2, S_NEXC = 3, S_HB = 4, S_CR = 5, S_CX = 6, S_DH = 7 };
 
-// Flags / counters shared by the step kernels (device memory, one per context).
+// Flags / counters shared by the step kernels (device memory, one per context). A run
+// numbers its steps k = 1, 2, ... from the state it starts from (k = 0); kernels of
+// step k use the buffers of index k mod 3 (rebuild requests, cell counts) or k mod 2
+// (positions, frames, cell members, step counters), so a kernel never clears or
+// overwrites what another workgroup of the same launch still reads.
 struct DevFlags {
-    uint32_t rebuild[2];     // neighbour rows must be rebuilt this step (by step parity)
+    uint32_t rebuild[3];     // neighbour rows must be rebuilt at step k (index k mod 3)
     uint32_t overflow;       // bit0: a cell overflowed, bit1: a neighbour row overflowed
-    uint32_t max_cell;       // largest cell occupancy seen
-    uint32_t max_neigh;      // largest neighbour count seen
-    unsigned long long step; // integration step counter (random stream)
+    uint32_t max_cell;       // largest cell occupancy seen (run steps: only when a cell overflowed)
+    uint32_t max_neigh;      // largest neighbour count seen (run steps: only when a row overflowed)
+    unsigned long long step; // integration steps done (kept by the host between runs)
     unsigned long long builds;
+    // random-stream counter of step k (index k mod 2): every thread of a step kernel
+    // reads it, so it has a cache line of its own, away from the atomics above
+    alignas(128) unsigned long long stepc[2];
 };
 
 __device__ __forceinline__ int type_of(double w) { return (int)(uint32_t)(__double_as_longlong(w) & 0xffffffffull); }
@@ -127,7 +134,7 @@ struct CellArgs {
     int nc[3];
     uint32_t cap;
     uint32_t* count;    // NULL: no cell list (no nonbonded term)
-    uint32_t* count_prev;  // step kernels: the other step parity's counts (zeroed as they go)
+    uint32_t* count_prev;  // step kernels: the counts of two steps back (zeroed as they go)
     uint32_t* members;
     DevFlags* flags;
 };
@@ -177,6 +184,36 @@ struct NbrArgs {
     int force_rebuild;     // rebuild regardless of the request
 };
 
+// First half of step t of a run for every particle (and, for the kernels that also
+// finish step t - 1, the Langevin parameters and the random-stream counter of that step).
+struct StepArgs {
+    uint32_t N;
+    double4* bd;       // Langevin force (xyz) of the last second half-kick, lab frame
+    double4* bd_tq;    // Langevin torque (xyz) of the last second half-kick, lab frame
+    const double4* pos_in;  // positions after step t - 1
+    double4* pos_out;       // positions of step t; its old content (step t - 2) names the
+                            // cells to clear in cells.count_prev
+    double4* vel;
+    double4* quat;
+    double4* angmom;
+    const float4* force;
+    const float4* torque;
+    const double* charge;
+    float4* fr0;       // frames of step t
+    float4* fr1;
+    float4* fr2;
+    const double4* pos_last;
+    Langevin lv;
+    double L[3], Linv[3];
+    double skin_half2;
+    unsigned long long seed;
+    DevFlags* flags;
+    uint32_t* rebuild_set;                 // rebuild request of step t
+    const unsigned long long* step_in;     // random-stream counter of step t - 1
+    unsigned long long* step_out;          // ... of step t (step_in + 1)
+    CellArgs cells;    // count: step t's cells; count_prev: step t - 2's, cleared as its particles leave
+};
+
 struct ForceArgs {
     uint32_t N;
     const double4* pos;    // x, y, z, type (low 32 bits of .w)
@@ -200,41 +237,11 @@ struct ForceArgs {
     double4* split;        // optional per-particle (fene, bexc, stack) half energies (WANT_E)
     unsigned long long* counts;  // COUNT instantiation: per term (inside cut-off, nonzero)
     NbrArgs nb;
-    // second half of the Langevin step, fused into the force kernel's epilogue
+    uint32_t* rebuild_clear;  // run steps: the request of step k + 2, cleared for step k + 1 to set
+    // 0: forces only; 1: then the Langevin second half-kick of step k (st.step_in); 2: then
+    // also the first half of step k + 1 (st), for the particle's own lane
     int integrate;
-    Langevin lv;
-    unsigned long long seed;
-    const DevFlags* sflags;  // step counter (random stream)
-    const double4* quat;
-    double4* vel;
-    double4* angmom;
-    double4* bd;
-    double4* bd_tq;
-};
-
-struct StepArgs {
-    uint32_t N;
-    double4* bd;       // Langevin force (xyz) of the last second half-kick, lab frame
-    double4* bd_tq;    // Langevin torque (xyz) of the last second half-kick, lab frame
-    double4* pos;
-    double4* vel;
-    double4* quat;
-    double4* angmom;
-    const float4* force;
-    const float4* torque;
-    const double* charge;
-    float4* fr0;
-    float4* fr1;
-    float4* fr2;
-    const double4* pos_last;
-    double dt, kT, gamma, gamma_r, mass, inertia;
-    double L[3], Linv[3];
-    double skin_half2;
-    unsigned long long seed;
-    DevFlags* flags;
-    uint32_t* rebuild_set;    // this step's rebuild request
-    uint32_t* rebuild_clear;  // the other parity's request, cleared for the next step
-    CellArgs cells;
+    StepArgs st;
 };
 
 }  // namespace oxd
