@@ -125,36 +125,37 @@ __device__ __forceinline__ uint32_t scan_cells(const NbrArgs& nb, const double4&
 }
 
 // ------------------------------------------------------------------ integrator
-// First half of step t for particle i (lv_first_half) from its state after step t - 1
-// (x, v, q, p), that step's force and torque (f, tq) and Langevin force and torque
-// (bd, bdt); then the frames of step t, its cell binning and the skin test (against
-// plast, the position of the last row rebuild) that asks for a rebuild. xold: what
-// pos_out[i] holds (the position two steps back); charge: the particle's charge. The translation, the skin test and the cell atomic come
-// first, so the atomic's round trip overlaps the long FP64 rotation chain. Only a cell
-// overflow touches the flags (overflow bit, largest occupancy for the resize): device-
-// scope atomics on one address from every wavefront would serialise the launch.
-__device__ __forceinline__ void first_half_particle(const StepArgs& a, uint32_t i, double4 x, double4 v,
-                                                    double4 q, double4 p, const float4& f, const float4& tq,
-                                                    const double4& bd, const double4& bdt, const double4& xold,
-                                                    const double4& plast, float charge) {
-    if (a.cells.count_prev) {
-        // the cell this particle was binned into two steps back: clear it for step
-        // t + 1, which bins into that buffer
-        a.cells.count_prev[cell_index(xold.x, xold.y, xold.z, a.cells.Linv, a.cells.nc)] = 0u;
-    }
-    lv_first_half_translate(a.lv, a.L, a.Linv, f.x, f.y, f.z, bd, x, v);
-    a.pos_out[i] = x;
-    a.vel[i] = v;
-    uint32_t cell = 0, slot = 0;
-    if (a.cells.count) {
-        cell = cell_index(x.x, x.y, x.z, a.cells.Linv, a.cells.nc);
-        slot = atomicAdd(&a.cells.count[cell], 1u);
-        const double dx = min_image(x.x - plast.x, a.L[0], a.Linv[0]);
-        const double dy = min_image(x.y - plast.y, a.L[1], a.Linv[1]);
-        const double dz = min_image(x.z - plast.z, a.L[2], a.Linv[2]);
-        if (dx * dx + dy * dy + dz * dz > a.skin_half2) atomicOr(a.rebuild_set, 1u);
-    }
-    lv_first_half_rotate(a.lv, tq.x, tq.y, tq.z, bdt, q, p);
+// Pieces of the first half of step t for particle i, shared by the kernels that run it:
+// step_clear zeroes, in the counts of step t - 2, the cell the particle was binned into
+// then (xold: its position then, what pos_out[i] still holds), for step t + 1 to bin
+// into; step_bin bins the new position x (the atomic's slot is consumed by step_finish,
+// so its round trip overlaps the rotation) and runs the skin test against plast (the
+// position of the last row rebuild); step_finish stores the rotation, the frames of
+// step t and the cell member. Only a cell overflow touches the flags (overflow bit,
+// largest occupancy for the resize): device-scope atomics on one address from every
+// wavefront would serialise the launch.
+__device__ __forceinline__ void step_clear(const StepArgs& a, const double4& xold) {
+    if (a.cells.count_prev) a.cells.count_prev[cell_index(xold.x, xold.y, xold.z, a.cells.Linv, a.cells.nc)] = 0u;
+}
+
+struct CellSlot {
+    uint32_t cell, slot;
+};
+
+__device__ __forceinline__ CellSlot step_bin(const StepArgs& a, const double4& x, const double4& plast) {
+    CellSlot b{0u, 0u};
+    if (!a.cells.count) return b;
+    b.cell = cell_index(x.x, x.y, x.z, a.cells.Linv, a.cells.nc);
+    b.slot = atomicAdd(&a.cells.count[b.cell], 1u);
+    const double dx = min_image(x.x - plast.x, a.L[0], a.Linv[0]);
+    const double dy = min_image(x.y - plast.y, a.L[1], a.Linv[1]);
+    const double dz = min_image(x.z - plast.z, a.L[2], a.Linv[2]);
+    if (dx * dx + dy * dy + dz * dz > a.skin_half2) atomicOr(a.rebuild_set, 1u);
+    return b;
+}
+
+__device__ __forceinline__ void step_finish(const StepArgs& a, uint32_t i, const double4& x, const double4& q,
+                                            const double4& p, CellSlot b, float charge) {
     a.quat[i] = q;
     a.angmom[i] = p;
     float4 f0, f1, f2;
@@ -164,40 +165,63 @@ __device__ __forceinline__ void first_half_particle(const StepArgs& a, uint32_t 
     f2.w = 0.f;
     a.fr0[i] = f0; a.fr1[i] = f1; a.fr2[i] = f2;
     if (!a.cells.count) return;
-    if (slot < a.cells.cap) {
-        a.cells.members[(size_t)cell * a.cells.cap + slot] = i;
+    if (b.slot < a.cells.cap) {
+        a.cells.members[(size_t)b.cell * a.cells.cap + b.slot] = i;
     } else {
         atomicOr(&a.flags->overflow, 1u);
-        atomicMax(&a.flags->max_cell, slot + 1);
+        atomicMax(&a.flags->max_cell, b.slot + 1);
     }
 }
 
-// First half of step t from the stored state (the first step of a run segment).
+// Second half of step t - 1 merged with the first half of step t (lv_merge_kick after
+// lv_merge_pre: v_pre, p_pre), with the force and torque of step t - 1: kick, drift,
+// binning and skin test, NO_SQUISH rotation, frames.
+__device__ __forceinline__ void merged_particle(const StepArgs& a, uint32_t i, double4 x, double4 v, const double4& q0,
+                                                double4 p, const float4& f, const float4& tq, const double4& plast,
+                                                float charge) {
+    double4 q = q0;
+    lv_merge_kick(a.lv, f.x, f.y, f.z, tq.x, tq.y, tq.z, q, v, p);
+    lv_drift(a.lv, a.L, a.Linv, v, x);
+    a.pos_out[i] = x;
+    a.vel[i] = v;
+    const CellSlot b = step_bin(a, x, plast);
+    lv_rotate(a.lv, q, p);
+    step_finish(a, i, x, q, p, b, charge);
+}
+
+// First half of step t from the stored state (the first step of a run chunk), with the
+// Langevin force and torque of the stored second half.
 __global__ void k_step1(StepArgs a) {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i == 0) *a.step_out = *a.step_in + 1ull;
-    if (i < a.N)
-        first_half_particle(a, i, a.pos_in[i], a.vel[i], a.quat[i], a.angmom[i], a.force[i], a.torque[i], a.bd[i],
-                            a.bd_tq[i], a.pos_out[i], a.pos_last[i], a.charge ? (float)a.charge[i] : 1.f);
+    if (i >= a.N) return;
+    double4 x = a.pos_in[i], v = a.vel[i], q = a.quat[i], p = a.angmom[i];
+    const float4 f = a.force[i], t = a.torque[i];
+    step_clear(a, a.pos_out[i]);
+    lv_first_half_translate(a.lv, a.L, a.Linv, f.x, f.y, f.z, a.bd[i], x, v);
+    a.pos_out[i] = x;
+    a.vel[i] = v;
+    const CellSlot b = step_bin(a, x, a.pos_last[i]);
+    lv_first_half_rotate(a.lv, t.x, t.y, t.z, a.bd_tq[i], q, p);
+    step_finish(a, i, x, q, p, b, a.charge ? (float)a.charge[i] : 1.f);
 }
 
-// Second half of step t - 1 (lv_second_half) then first half of step t: the kernel
-// between two force launches on systems large enough to fill the GPU (smaller ones run
-// both halves in the force kernel's epilogue).
+// Second half of step t - 1 and first half of step t, merged: the kernel between two
+// force launches on systems large enough to fill the GPU (smaller ones run it in the
+// force kernel's epilogue).
 __global__ void k_step21(StepArgs a) {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i == 0) *a.step_out = *a.step_in + 1ull;
-    if (i < a.N) {
-        double4 v = a.vel[i], p = a.angmom[i], bd, bdt;
-        const double4 q = a.quat[i];
-        const float4 f = a.force[i], t = a.torque[i];
-        lv_second_half(a.lv, i, *a.step_in, a.seed, f.x, f.y, f.z, t.x, t.y, t.z, q, v, p, bd, bdt);
-        first_half_particle(a, i, a.pos_in[i], v, q, p, f, t, bd, bdt, a.pos_out[i], a.pos_last[i],
-                            a.charge ? (float)a.charge[i] : 1.f);
-    }
+    if (i >= a.N) return;
+    double4 v = a.vel[i], p = a.angmom[i];
+    const double4 q = a.quat[i];
+    step_clear(a, a.pos_out[i]);
+    lv_merge_pre(a.lv, i, *a.step_in, a.seed, q, v, p);
+    merged_particle(a, i, a.pos_in[i], v, q, p, a.force[i], a.torque[i], a.pos_last[i],
+                    a.charge ? (float)a.charge[i] : 1.f);
 }
 
-// Second half of step t - 1 alone (the last step of a run segment, large systems).
+// Second half of step t - 1 alone (the last step of a run chunk, large systems).
 __global__ void k_step2(StepArgs a) {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= a.N) return;
@@ -228,10 +252,14 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
     const uint32_t* row = nullptr;
     double4 pi = make_double4(0, 0, 0, 0);
     if (active) pi = a.pos[i];
-    double4 e_v, e_p, e_q, e_xold, e_plast;
+    // FUSED: the particle's integrator state, and the part of the merged half-kicks that
+    // needs no force (noise, drag), ahead of the pair loop
+    double4 e_v, e_p, e_q, e_plast;
     if (FUSED && active && lane == 0) {
         const StepArgs& s = a.st;
-        e_v = s.vel[i]; e_p = s.angmom[i]; e_q = s.quat[i]; e_xold = s.pos_out[i]; e_plast = s.pos_last[i];
+        e_v = s.vel[i]; e_p = s.angmom[i]; e_q = s.quat[i]; e_plast = s.pos_last[i];
+        step_clear(s, s.pos_out[i]);
+        lv_merge_pre(s.lv, i, *s.step_in, s.seed, e_q, e_v, e_p);
     }
     if (rebuild) {
         // ---- neighbour row of i (uniform branch: every thread of the block is here)
@@ -385,14 +413,11 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
         float4 fo = make_float4(F.x, F.y, F.z, Eh);
         float4 to = make_float4(T.x, T.y, T.z, 0.f);
         if (FUSED) {
-            // Langevin second half of step k with the force just computed, then the first
-            // half of step k + 1: both touch only particle i's own state, and the positions
-            // and frames of step k + 1 go to the other buffer of each pair, so no workgroup
-            // still reading step k sees them
-            const StepArgs& s = a.st;
-            double4 v = e_v, p = e_p, bd, bdt;
-            lv_second_half(s.lv, i, *s.step_in, s.seed, fo.x, fo.y, fo.z, to.x, to.y, to.z, e_q, v, p, bd, bdt);
-            first_half_particle(s, i, pi, v, e_q, p, fo, to, bd, bdt, e_xold, rebuild ? pi : e_plast, charge_i);
+            // Langevin second half of step k with the force just computed, merged with the
+            // first half of step k + 1: both touch only particle i's own state, and the
+            // positions and frames of step k + 1 go to the other buffer of each pair, so no
+            // workgroup still reading step k sees them
+            merged_particle(a.st, i, pi, e_v, e_q, e_p, fo, to, rebuild ? pi : e_plast, charge_i);
         } else {
             if (a.accumulate) {
                 float4 f = a.force[i], t = a.torque[i];

@@ -33,31 +33,19 @@ __device__ __forceinline__ double4 qmul(double4 a, double4 b) {
                         a.x * b.w + a.y * b.z - a.z * b.y + a.w * b.x);
 }
 __device__ __forceinline__ double4 qconj(double4 a) { return make_double4(a.x, -a.y, -a.z, -a.w); }
-// Rotation matrix of a unit quaternion: lab = R body (columns: the body axes a1, a2, a3).
-struct Rot {
-    double m[9];
-};
-__device__ __forceinline__ Rot rot_of(const double4& q) {
-    const double w = q.x, x = q.y, y = q.z, z = q.w;
-    Rot r;
-    r.m[0] = 1.0 - 2.0 * (y * y + z * z); r.m[1] = 2.0 * (x * y - z * w);       r.m[2] = 2.0 * (x * z + y * w);
-    r.m[3] = 2.0 * (x * y + z * w);       r.m[4] = 1.0 - 2.0 * (x * x + z * z); r.m[5] = 2.0 * (y * z - x * w);
-    r.m[6] = 2.0 * (x * z - y * w);       r.m[7] = 2.0 * (y * z + x * w);       r.m[8] = 1.0 - 2.0 * (x * x + y * y);
-    return r;
+// (0, w) (x) q and q (x) (0, w) for a pure vector w. For a unit q, q (x) (0, R^T w) =
+// (0, w) (x) q: the body-frame torque kick of NO_SQUISH needs no rotation matrix.
+__device__ __forceinline__ double4 qmul_vq(double wx, double wy, double wz, const double4& q) {
+    return make_double4(-wx * q.y - wy * q.z - wz * q.w,
+                        wx * q.x + wy * q.w - wz * q.z,
+                        -wx * q.w + wy * q.x + wz * q.y,
+                        wx * q.z - wy * q.y + wz * q.x);
 }
-// lab vector v into the body frame (R^T v = conj(q) v q for a unit q)
-__device__ __forceinline__ void to_body(const Rot& r, double vx, double vy, double vz, double& bx, double& by,
-                                        double& bz) {
-    bx = r.m[0] * vx + r.m[3] * vy + r.m[6] * vz;
-    by = r.m[1] * vx + r.m[4] * vy + r.m[7] * vz;
-    bz = r.m[2] * vx + r.m[5] * vy + r.m[8] * vz;
-}
-// body vector v into the lab frame (R v = q v conj(q))
-__device__ __forceinline__ void to_lab(const Rot& r, double vx, double vy, double vz, double& lx, double& ly,
-                                       double& lz) {
-    lx = r.m[0] * vx + r.m[1] * vy + r.m[2] * vz;
-    ly = r.m[3] * vx + r.m[4] * vy + r.m[5] * vz;
-    lz = r.m[6] * vx + r.m[7] * vy + r.m[8] * vz;
+__device__ __forceinline__ double4 qmul_qv(const double4& q, double wx, double wy, double wz) {
+    return make_double4(-q.y * wx - q.z * wy - q.w * wz,
+                        q.x * wx + q.z * wz - q.w * wy,
+                        q.x * wy - q.y * wz + q.w * wx,
+                        q.x * wz + q.y * wy - q.z * wx);
 }
 // cos and sin of x: Taylor series through x^7 / x^6 for |x| <= 2^-5 (the first dropped
 // terms, x^9/9! and x^8/8!, are below 2.5e-18 and 2.3e-17 relative, under half an FP64
@@ -128,6 +116,16 @@ __device__ __forceinline__ double rsqrt_near1(double s) {
     return 1.0 / sqrt(s);
 }
 
+// x += dt v, wrapped into the box centred on the origin.
+__device__ __forceinline__ void lv_drift(const Langevin& lv, const double L[3], const double Linv[3], const double4& v,
+                                         double4& x) {
+    const double dt = lv.dt;
+    x.x += dt * v.x; x.y += dt * v.y; x.z += dt * v.z;
+    x.x -= L[0] * floor(x.x * Linv[0] + 0.5);
+    x.y -= L[1] * floor(x.y * Linv[1] + 0.5);
+    x.z -= L[2] * floor(x.z * Linv[2] + 0.5);
+}
+
 // First half of a step for one particle: v += dt/2 (F + F_L)/m, x += dt v (wrapped
 // into the box centred on the origin), p += dt q (x) (tau + tau_L)_body, NO_SQUISH
 // rotation (axis 3: dt/2, 2: dt/2, 1: dt, 2: dt/2, 3: dt/2), renormalised q.
@@ -137,23 +135,14 @@ __device__ __forceinline__ double rsqrt_near1(double s) {
 __device__ __forceinline__ void lv_first_half_translate(const Langevin& lv, const double L[3], const double Linv[3],
                                                         float fx, float fy, float fz, const double4& bd,
                                                         double4& x, double4& v) {
-    const double dt = lv.dt;
     const double hm = lv.hm;
     v.x += hm * (fx + bd.x); v.y += hm * (fy + bd.y); v.z += hm * (fz + bd.z);
-    x.x += dt * v.x; x.y += dt * v.y; x.z += dt * v.z;
-    x.x -= L[0] * floor(x.x * Linv[0] + 0.5);
-    x.y -= L[1] * floor(x.y * Linv[1] + 0.5);
-    x.z -= L[2] * floor(x.z * Linv[2] + 0.5);
+    lv_drift(lv, L, Linv, v, x);
 }
 
-// The rotational part of lv_first_half (kick, NO_SQUISH rotation, renormalisation).
-__device__ __forceinline__ void lv_first_half_rotate(const Langevin& lv, float tx, float ty, float tz,
-                                                     const double4& bdt, double4& q, double4& p) {
+// NO_SQUISH free rotation (axis 3: dt/2, 2: dt/2, 1: dt, 2: dt/2, 3: dt/2) and renormalisation.
+__device__ __forceinline__ void lv_rotate(const Langevin& lv, double4& q, double4& p) {
     const double dt = lv.dt;
-    double tbx, tby, tbz;
-    to_body(rot_of(q), tx + bdt.x, ty + bdt.y, tz + bdt.z, tbx, tby, tbz);
-    double4 kick = qmul(q, make_double4(0.0, tbx, tby, tbz));
-    p.x += dt * kick.x; p.y += dt * kick.y; p.z += dt * kick.z; p.w += dt * kick.w;
     const double inv4I = lv.inv4I;
     rotor(3, 0.5 * dt, inv4I, q, p);
     rotor(2, 0.5 * dt, inv4I, q, p);
@@ -164,6 +153,16 @@ __device__ __forceinline__ void lv_first_half_rotate(const Langevin& lv, float t
     q.x *= nq; q.y *= nq; q.z *= nq; q.w *= nq;
 }
 
+// The rotational part of lv_first_half (kick, NO_SQUISH rotation, renormalisation).
+__device__ __forceinline__ void lv_first_half_rotate(const Langevin& lv, float tx, float ty, float tz,
+                                                     const double4& bdt, double4& q, double4& p) {
+    const double dt = lv.dt;
+    // p += dt q (x) (0, tau_body) = dt (0, tau_lab) (x) q
+    const double4 kick = qmul_vq(tx + bdt.x, ty + bdt.y, tz + bdt.z, q);
+    p.x += dt * kick.x; p.y += dt * kick.y; p.z += dt * kick.z; p.w += dt * kick.w;
+    lv_rotate(lv, q, p);
+}
+
 __device__ __forceinline__ void lv_first_half(const Langevin& lv, const double L[3], const double Linv[3],
                                               float fx, float fy, float fz, float tx, float ty, float tz,
                                               const double4& bd, const double4& bdt,
@@ -172,46 +171,73 @@ __device__ __forceinline__ void lv_first_half(const Langevin& lv, const double L
     lv_first_half_rotate(lv, tx, ty, tz, bdt, q, p);
 }
 
-// Second half for particle `i` at integration step `step` with random key `seed`:
-// Langevin force (drag -gamma v plus uniform noise of variance 2 gamma kT / dt per
-// component) and torque (drag -gamma_r L / I plus Gaussian noise of variance
-// 2 gamma_r kT / dt, body frame), then v += dt/2 (F + F_L)/m, p += dt q (x) (tau + tau_L).
-// Returns the Langevin force and torque (lab frame) in bd / bdt for the next step.
-__device__ __forceinline__ void lv_second_half(const Langevin& lv, uint32_t i, unsigned long long step,
-                                               unsigned long long seed, float fx, float fy, float fz,
-                                               float tx, float ty, float tz, const double4& q,
-                                               double4& v, double4& p, double4& bd, double4& bdt) {
-    const double dt = lv.dt;
+// Langevin force and torque of the second half-kick of step `step` for particle `i`
+// with random key `seed`, from the state (v, q, p) before that kick: b = -gamma v plus
+// uniform noise of variance 2 gamma kT / dt per component (lab frame); rb = -gamma_r
+// L_body / I plus Gaussian noise of variance 2 gamma_r kT / dt (body frame).
+__device__ __forceinline__ void lv_noise(const Langevin& lv, uint32_t i, unsigned long long step,
+                                         unsigned long long seed, const double4& v, const double4& q,
+                                         const double4& p, double b[3], double rb[3]) {
     uint32_t c[4] = {i, (uint32_t)step, (uint32_t)(step >> 32), 0u};
     philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
     uint32_t d[4] = {i, (uint32_t)step, (uint32_t)(step >> 32), 1u};
     philox(d, (uint32_t)seed, (uint32_t)(seed >> 32));
     const double ct = lv.ct;
-    double bx = -lv.gamma * v.x + ct * (2.0 * u01f(c[0]) - 1.0);
-    double by = -lv.gamma * v.y + ct * (2.0 * u01f(c[1]) - 1.0);
-    double bz = -lv.gamma * v.z + ct * (2.0 * u01f(c[2]) - 1.0);
-    const double hm = lv.hm;
-    v.x += hm * (fx + bx); v.y += hm * (fy + by); v.z += hm * (fz + bz);
-    bd = make_double4(bx, by, bz, 0.0);
-    const Rot R = rot_of(q);
-    double tbx, tby, tbz;
-    to_body(R, tx, ty, tz, tbx, tby, tbz);
-    double4 Lb = qmul(qconj(q), p);  // = 2 (0, L_body)
+    b[0] = -lv.gamma * v.x + ct * (2.0 * u01f(c[0]) - 1.0);
+    b[1] = -lv.gamma * v.y + ct * (2.0 * u01f(c[1]) - 1.0);
+    b[2] = -lv.gamma * v.z + ct * (2.0 * u01f(c[2]) - 1.0);
+    const double4 Lb = qmul(qconj(q), p);  // = (0, 2 L_body)
     const double cr = lv.cr, gr = lv.gr_half_I;
     // Box-Muller in FP32: the noise needs no FP64 accuracy
     float r1 = sqrtf(-2.f * __logf(u01f(c[3]))), r2 = sqrtf(-2.f * __logf(u01f(d[1])));
     float s0, c0, s2, c2;
     __sincosf(6.2831853f * u01f(d[0]), &s0, &c0);
     __sincosf(6.2831853f * u01f(d[2]), &s2, &c2);
-    double rbx = cr * (double)(r1 * c0) - gr * Lb.y;
-    double rby = cr * (double)(r1 * s0) - gr * Lb.z;
-    double rbz = cr * (double)(r2 * c2) - gr * Lb.w;
-    tbx += rbx; tby += rby; tbz += rbz;
-    double lx, ly, lz;
-    to_lab(R, rbx, rby, rbz, lx, ly, lz);
-    bdt = make_double4(lx, ly, lz, 0.0);
-    double4 kick = qmul(q, make_double4(0.0, tbx, tby, tbz));
-    p.x += dt * kick.x; p.y += dt * kick.y; p.z += dt * kick.z; p.w += dt * kick.w;
+    rb[0] = cr * (double)(r1 * c0) - gr * Lb.y;
+    rb[1] = cr * (double)(r1 * s0) - gr * Lb.z;
+    rb[2] = cr * (double)(r2 * c2) - gr * Lb.w;
+}
+
+// Second half of step `step` for particle `i`: v += dt/2 (F + b)/m, p += dt q (x) (tau_body
+// + rb), with (b, rb) from lv_noise. Returns the Langevin force and torque (lab frame) in
+// bd / bdt for the next step's first half.
+__device__ __forceinline__ void lv_second_half(const Langevin& lv, uint32_t i, unsigned long long step,
+                                               unsigned long long seed, float fx, float fy, float fz,
+                                               float tx, float ty, float tz, const double4& q,
+                                               double4& v, double4& p, double4& bd, double4& bdt) {
+    const double dt = lv.dt;
+    double b[3], rb[3];
+    lv_noise(lv, i, step, seed, v, q, p, b, rb);
+    const double hm = lv.hm;
+    v.x += hm * (fx + b[0]); v.y += hm * (fy + b[1]); v.z += hm * (fz + b[2]);
+    bd = make_double4(b[0], b[1], b[2], 0.0);
+    const double4 K = qmul_qv(q, rb[0], rb[1], rb[2]);  // q (x) (0, rb)
+    const double4 T = qmul_vq(tx, ty, tz, q);            // q (x) (0, R^T tau)
+    p.x += dt * (T.x + K.x); p.y += dt * (T.y + K.y); p.z += dt * (T.z + K.z); p.w += dt * (T.w + K.w);
+    const double4 l = qmul(K, qconj(q));  // (0, R rb): the Langevin torque in the lab frame
+    bdt = make_double4(l.y, l.z, l.w, 0.0);
+}
+
+// The second half of step k merged with the first half of step k + 1 (velocity Verlet
+// as leapfrog: the two half-kicks with the same force add up), in two parts. lv_merge_pre
+// needs only the state before the force evaluation and may run ahead of it: with (b, rb)
+// of step k, v += dt b / m and p += 2 dt q (x) (0, rb). lv_merge_kick then adds the force
+// and torque of step k: v += dt F / m, p += 2 dt (0, tau_lab) (x) q.
+__device__ __forceinline__ void lv_merge_pre(const Langevin& lv, uint32_t i, unsigned long long step,
+                                             unsigned long long seed, const double4& q, double4& v, double4& p) {
+    double b[3], rb[3];
+    lv_noise(lv, i, step, seed, v, q, p, b, rb);
+    const double h = 2.0 * lv.hm, h2 = 2.0 * lv.dt;
+    const double4 K = qmul_qv(q, rb[0], rb[1], rb[2]);
+    v.x += h * b[0]; v.y += h * b[1]; v.z += h * b[2];
+    p.x += h2 * K.x; p.y += h2 * K.y; p.z += h2 * K.z; p.w += h2 * K.w;
+}
+__device__ __forceinline__ void lv_merge_kick(const Langevin& lv, float fx, float fy, float fz, float tx, float ty,
+                                              float tz, const double4& q, double4& v, double4& p) {
+    const double h = 2.0 * lv.hm, h2 = 2.0 * lv.dt;
+    v.x += h * fx; v.y += h * fy; v.z += h * fz;
+    const double4 T = qmul_vq(tx, ty, tz, q);
+    p.x += h2 * T.x; p.y += h2 * T.y; p.z += h2 * T.z; p.w += h2 * T.w;
 }
 
 }  // namespace oxd
