@@ -232,28 +232,68 @@ __global__ void k_step2(StepArgs a) {
 }
 
 // ------------------------------------------------------------------ fused forces
+// A workgroup of 256 threads owns P consecutive particles (TPP = 256 / P lanes each in
+// the per-particle phases). (1) Each particle's neighbour row, rebuilt in-kernel when the
+// skin test asked for it; its leading class-0 entries are "near" (every term), the rest
+// "far" (excluded volume and Debye-Hueckel at most). (2) The workgroup's near entries,
+// its bond slots and its far entries, laid end to end with each part starting on a
+// wavefront boundary, are evaluated one per thread, 256 per round, so the base-base,
+// coaxial and bonded paths run on full wavefronts instead of on the few lanes of each
+// particle that hold such a pair; each evaluation leaves its force, torque and energy in
+// an LDS record. (3) Each particle's lanes sum its records in a fixed order (runs are
+// bit-reproducible), then store, or (FUSED) integrate.
+constexpr int kRecBase = 7;  // F (3), T (3), half energy
+
+// Largest s in [0, P) with off[s] <= x (off ascending, off[0] = 0).
+template<int P>
+__device__ __forceinline__ uint32_t owner_of(const uint32_t* off, uint32_t x) {
+    uint32_t lo = 0, n = P;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (n <= 1) break;
+        const uint32_t h = n / 2;
+        if (off[lo + h] <= x) lo += h;
+        n -= h;
+    }
+    return lo;
+}
+
 // FUSED: a run step whose epilogue also takes the particle through the Langevin second
 // half of step k and the first half of step k + 1 (a.integrate == 2); its loads are
 // issued before the pair loop, and the force and torque, consumed in the epilogue, are
 // not stored.
-template<int TPP, bool WANT_E, bool WANT_V, bool COUNT, bool FUSED = false>
+template<int P, bool WANT_E, bool WANT_V, bool COUNT, bool FUSED = false>
 __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
-    extern __shared__ uint32_t lds_rows[];
-    const uint32_t tid = xcd_block() * blockDim.x + threadIdx.x;
-    const uint32_t i = tid / TPP;
-    const uint32_t lane = tid % TPP;
+    constexpr uint32_t TPP = 256 / P;
+    constexpr int NREC = kRecBase + (WANT_V ? 6 : 0) + (WANT_E ? 3 : 0);
+    constexpr int RV = kRecBase, RS = kRecBase + (WANT_V ? 6 : 0);  // virial, split energies
+    __shared__ double4 s_pos[P];
+    __shared__ float4 s_fr[3][P];
+    __shared__ size_t s_head[P];
+    __shared__ uint32_t s_nnear[P];
+    __shared__ uint32_t s_off[2][P + 1];  // exclusive prefix sums of near / far counts
+    __shared__ float s_rec[NREC][256];
+    // The term parameters, staged in LDS: the inlined term evaluators reach them through
+    // addresses the compiler may select per lane (one code path serving the hydrogen-
+    // bonding and cross-stacking halves), which from global memory would be a vector
+    // load, with its latency, per parameter.
+    __shared__ uint4 s_par[(sizeof(DParams) + 15) / 16];
+    extern __shared__ uint32_t lds_rows[];  // rebuild launches: 2 rows of max_neigh per particle
+
+    const uint32_t t = threadIdx.x;
+    const uint32_t slot = t / TPP, lane = t % TPP;
+    const uint32_t i = xcd_block() * P + slot;
     const bool active = i < a.N;
-    const DParams& P = *a.P;
+    for (uint32_t w = t; w < (sizeof(DParams) + 15) / 16; w += 256) s_par[w] = ((const uint4*)a.P)[w];
+    const DParams& P_ = *(const DParams*)s_par;
     const uint32_t mask = a.mask;
     const NbrArgs& nb = a.nb;
     const bool rebuild = nb.force_rebuild || (nb.rebuild_flag && *nb.rebuild_flag);
 
-    uint32_t nn = 0;
-    const uint32_t* row = nullptr;
     double4 pi = make_double4(0, 0, 0, 0);
     if (active) pi = a.pos[i];
     // FUSED: the particle's integrator state, and the part of the merged half-kicks that
-    // needs no force (noise, drag), ahead of the pair loop
+    // needs no force (noise, drag), while the first loads are in flight
     double4 e_v, e_p, e_q, e_plast;
     if (FUSED && active && lane == 0) {
         const StepArgs& s = a.st;
@@ -261,11 +301,14 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
         step_clear(s, s.pos_out[i]);
         lv_merge_pre(s.lv, i, *s.step_in, s.seed, e_q, e_v, e_p);
     }
+
+    // ---- (1) rows
+    uint32_t nn = 0, nnear = 0;
+    const uint32_t* srow = nullptr;
     if (rebuild) {
-        // ---- neighbour row of i (uniform branch: every thread of the block is here)
-        const uint32_t slot = threadIdx.x / TPP;
+        // neighbour row of i (uniform branch: every thread of the block is here)
         uint32_t* urow = lds_rows + (size_t)slot * 2 * nb.max_neigh;
-        uint32_t* srow = urow + nb.max_neigh;
+        uint32_t* row = urow + nb.max_neigh;
         int b0 = -1, b1 = -1;
         uint32_t cnt = 0;
         if (active) {
@@ -274,144 +317,200 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
         }
         uint32_t v = cnt;
 #pragma unroll
-        for (int d = 1; d < TPP; d <<= 1) {
+        for (uint32_t d = 1; d < TPP; d <<= 1) {
             uint32_t y = __shfl_up(v, d, TPP);
-            if ((int)lane >= d) v += y;
+            if (lane >= d) v += y;
         }
         const uint32_t total = __shfl(v, TPP - 1, TPP);
         if (active) scan_cells<TPP>(nb, pi, i, b0, b1, lane, urow, v - cnt, true);
         __syncthreads();
         const uint32_t m = min(total, nb.max_neigh);
+        uint32_t near = 0;
         if (active) {
             for (uint32_t e = lane; e < m; e += TPP) {
                 uint32_t x = urow[e], r = 0;
                 // ties (never expected) break by position, so every rank is distinct
                 for (uint32_t q = 0; q < m; ++q) r += (urow[q] < x || (urow[q] == x && q < e)) ? 1u : 0u;
-                srow[r] = x;
+                row[r] = x;
                 a.nlist[a.head[i] + r] = x;
-            }
-            if (lane == 0) {
-                a.n_neigh[i] = m;
-                nb.pos_last[i] = pi;
-                if (total > nb.max_neigh) atomicOr(&nb.flags->overflow, 2u);
-                // the largest count sizes the rows: wanted from a forced rebuild (which
-                // sizes them with headroom) and after an overflow; per-particle atomics
-                // on one address would serialise the ordinary rebuild steps
-                if (nb.force_rebuild || total > nb.max_neigh) atomicMax(&nb.flags->max_neigh, total);
-                if (i == 0) atomicAdd(&nb.flags->builds, 1ull);
+                near += (x >> kClassShift) == 0u ? 1u : 0u;
             }
         }
-        __syncthreads();
+#pragma unroll
+        for (uint32_t d = TPP / 2; d > 0; d >>= 1) near += __shfl_xor(near, d, TPP);
+        if (active && lane == 0) {
+            a.n_neigh[i] = m;
+            if (a.n_near) a.n_near[i] = near;
+            nb.pos_last[i] = pi;
+            if (total > nb.max_neigh) atomicOr(&nb.flags->overflow, 2u);
+            // the largest count sizes the rows: wanted from a forced rebuild (which
+            // sizes them with headroom) and after an overflow; per-particle atomics
+            // on one address would serialise the ordinary rebuild steps
+            if (nb.force_rebuild || total > nb.max_neigh) atomicMax(&nb.flags->max_neigh, total);
+            if (i == 0) atomicAdd(&nb.flags->builds, 1ull);
+        }
         nn = m;
-        row = srow;
-    } else if (active) {
+        nnear = near;
+        srow = row;
+    } else if (active && lane == 0) {
         nn = a.n_neigh[i];
-        row = a.nlist + a.head[i];
+        nnear = a.n_near ? min(a.n_near[i], nn) : nn;  // rows without classes: all near
     }
-
-    V F = zero3(), T = zero3();
-    float Eh = 0.f;
     float charge_i = 1.f;
+    if (lane == 0) {
+        s_nnear[slot] = active ? nnear : 0u;
+        s_off[0][slot + 1] = active ? nnear : 0u;
+        s_off[1][slot + 1] = active ? nn - nnear : 0u;
+        if (active) {
+            s_pos[slot] = pi;
+            const float4 f0 = a.fr0[i];
+            s_fr[0][slot] = f0; s_fr[1][slot] = a.fr1[i]; s_fr[2][slot] = a.fr2[i];
+            s_head[slot] = rebuild ? (size_t)slot * 2 * nb.max_neigh + nb.max_neigh : a.head[i];
+            charge_i = f0.w;
+        }
+    }
+    __syncthreads();
+    if (t < 64) {  // exclusive prefix sums over the P particles (P <= 64: one wavefront)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            uint32_t v = t < P ? s_off[c][t + 1] : 0u;
+#pragma unroll
+            for (uint32_t d = 1; d < 64; d <<= 1) {
+                uint32_t y = __shfl_up(v, d, 64);
+                if (t >= d) v += y;
+            }
+            if (t < P) s_off[c][t + 1] = v;
+            if (t == 0) s_off[c][0] = 0u;
+        }
+    }
+    __syncthreads();
+    const uint32_t NN = s_off[0][P], NF = s_off[1][P];
+    const uint32_t NB = (mask & (1u << T_BONDED)) ? ((2u * P + 63u) & ~63u) : 0u;
+    const uint32_t B0 = (NN + 63u) & ~63u, F0 = B0 + NB, NT = F0 + NF;
+
+    // ---- (2) entries, 256 per round; (3) each particle's lanes sum its records
     float et[8];
-    float vir[6];
-#pragma unroll
-    for (int s = 0; s < 8; ++s) et[s] = 0.f;
-#pragma unroll
-    for (int s = 0; s < 6; ++s) vir[s] = 0.f;
     uint32_t cnt[14];
 #pragma unroll
-    for (int s = 0; s < 14; ++s) cnt[s] = 0u;
-    if (active) {
-        const float4 f0 = a.fr0[i], f1 = a.fr1[i], f2 = a.fr2[i];
-        Frame fi{mk(f0.x, f0.y, f0.z), mk(f1.x, f1.y, f1.z), mk(f2.x, f2.y, f2.z)};
-        const float qi = f0.w;
-        charge_i = qi;
-        const int ti = __float_as_int(f1.w);
-        const int nt = P.ntypes;
-        for (uint32_t k = lane; k < nn; k += TPP) {
-            const uint32_t j = row[k] & kIndexMask;
+    for (int k = 0; k < 8; ++k) et[k] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 14; ++k) cnt[k] = 0u;
+    float acc[NREC];
+#pragma unroll
+    for (int k = 0; k < NREC; ++k) acc[k] = 0.f;
+    const int nt = P_.ntypes;
+    for (uint32_t r0 = 0; r0 < NT; r0 += 256) {
+        const uint32_t e = r0 + t;
+        float rec[NREC];
+#pragma unroll
+        for (int k = 0; k < NREC; ++k) rec[k] = 0.f;
+        bool pair = false;
+        uint32_t s = 0, k = 0;
+        if (e < NN) {
+            s = owner_of<P>(s_off[0], e);
+            k = e - s_off[0][s];
+            pair = true;
+        } else if (e >= F0 && e < NT) {
+            s = owner_of<P>(s_off[1], e - F0);
+            k = s_nnear[s] + (e - F0 - s_off[1][s]);
+            pair = true;
+        }
+        if (pair) {
+            const uint32_t j = (srow ? lds_rows[s_head[s] + k] : a.nlist[s_head[s] + k]) & kIndexMask;
+            const double4 ps = s_pos[s];
+            const float4 f0 = s_fr[0][s], f1 = s_fr[1][s], f2 = s_fr[2][s];
             const double4 pj = a.pos[j];
             const float4 g0 = a.fr0[j], g1 = a.fr1[j], g2 = a.fr2[j];
+            Frame fi{mk(f0.x, f0.y, f0.z), mk(f1.x, f1.y, f1.z), mk(f2.x, f2.y, f2.z)};
             Frame fj{mk(g0.x, g0.y, g0.z), mk(g1.x, g1.y, g1.z), mk(g2.x, g2.y, g2.z)};
-            const int tj = __float_as_int(g1.w);
-            const int tp = ti * nt + tj;
-            V dr = mk((float)min_image(pi.x - pj.x, a.L[0], a.Linv[0]),
-                      (float)min_image(pi.y - pj.y, a.L[1], a.Linv[1]),
-                      (float)min_image(pi.z - pj.z, a.L[2], a.Linv[2]));
+            const int tp = __float_as_int(f1.w) * nt + __float_as_int(g1.w);
+            V dr = mk((float)min_image(ps.x - pj.x, a.L[0], a.Linv[0]),
+                      (float)min_image(ps.y - pj.y, a.L[1], a.Linv[1]),
+                      (float)min_image(ps.z - pj.z, a.L[2], a.Linv[2]));
             const float rsq = dot(dr, dr);
-            const V Rc = -dr;
-            V Fp = zero3();
-            const float ep = pair_terms<COUNT>(P, mask, tp, rsq, Rc, fi, fj, qi * g0.w, Fp, T, et, cnt);
-            F += Fp;
-            Eh += 0.5f * ep;
+            V Fp = zero3(), Tp = zero3();
+            const float ep = pair_terms<COUNT>(P_, mask, tp, rsq, -dr, fi, fj, f0.w * g0.w, Fp, Tp, et, cnt);
+            rec[0] = Fp.x; rec[1] = Fp.y; rec[2] = Fp.z;
+            rec[3] = Tp.x; rec[4] = Tp.y; rec[5] = Tp.z;
+            rec[6] = 0.5f * ep;
             if (WANT_V) {
-                vir[0] += 0.5f * dr.x * Fp.x; vir[1] += 0.5f * dr.x * Fp.y; vir[2] += 0.5f * dr.x * Fp.z;
-                vir[3] += 0.5f * dr.y * Fp.y; vir[4] += 0.5f * dr.y * Fp.z; vir[5] += 0.5f * dr.z * Fp.z;
+                rec[RV + 0] = 0.5f * dr.x * Fp.x; rec[RV + 1] = 0.5f * dr.x * Fp.y; rec[RV + 2] = 0.5f * dr.x * Fp.z;
+                rec[RV + 3] = 0.5f * dr.y * Fp.y; rec[RV + 4] = 0.5f * dr.y * Fp.z; rec[RV + 5] = 0.5f * dr.z * Fp.z;
             }
-        }
-        if (mask & (1u << T_BONDED)) {
-            // bond slots go to the last lanes, which hold the fewest pairs
-            for (uint32_t b = (TPP - 1) - lane; b < 2; b += TPP) {
-                const int other = a.bpart[2 * i + b];
-                if (other < 0) continue;
-                const int info = a.binfo[2 * i + b];
+        } else if (e >= B0 && e < B0 + 2u * P && NB) {
+            // bond slot (e - B0) & 1 of particle (e - B0) / 2
+            s = (e - B0) >> 1;
+            const uint32_t ib = xcd_block() * P + s;
+            const int other = ib < a.N ? a.bpart[2 * ib + ((e - B0) & 1u)] : -1;
+            if (other >= 0) {
+                const int info = a.binfo[2 * ib + ((e - B0) & 1u)];
                 const int role = info & 1;
                 const int bt = info >> 1;
-                const uint32_t A = role == 0 ? i : (uint32_t)other;
-                const uint32_t B = role == 0 ? (uint32_t)other : i;
+                const uint32_t A = role == 0 ? ib : (uint32_t)other;
+                const uint32_t B = role == 0 ? (uint32_t)other : ib;
                 const double4 pA = a.pos[A], pB = a.pos[B];
                 const float4 x0 = a.fr0[A], x1 = a.fr1[A], x2 = a.fr2[A];
                 const float4 y0 = a.fr0[B], y1 = a.fr1[B], y2 = a.fr2[B];
                 Frame fA{mk(x0.x, x0.y, x0.z), mk(x1.x, x1.y, x1.z), mk(x2.x, x2.y, x2.z)};
                 Frame fB{mk(y0.x, y0.y, y0.z), mk(y1.x, y1.y, y1.z), mk(y2.x, y2.y, y2.z)};
                 BondRes r;
-                bond_eval(P.bond[bt], min_image(pB.x - pA.x, a.L[0], a.Linv[0]),
+                bond_eval(P_.bond[bt], min_image(pB.x - pA.x, a.L[0], a.Linv[0]),
                           min_image(pB.y - pA.y, a.L[1], a.Linv[1]), min_image(pB.z - pA.z, a.L[2], a.Linv[2]),
                           fA, fB, __float_as_int(x1.w), __float_as_int(y1.w), r);
                 if (COUNT) { cnt[2 * T_BONDED]++; cnt[2 * T_BONDED + 1] += r.e_stack != 0.f; }
-                if (role == 0) { F -= r.FB; T += r.TA; }
-                else { F += r.FB; T += r.TB; }
+                V F = role == 0 ? -r.FB : r.FB;
+                V T = role == 0 ? r.TA : r.TB;
+                rec[0] = F.x; rec[1] = F.y; rec[2] = F.z;
+                rec[3] = T.x; rec[4] = T.y; rec[5] = T.z;
                 // per-term sums count every directed evaluation in full and are halved at
                 // the end (each pair and each bond is evaluated from both of its ends)
-                float ef = (float)r.e_fene;
+                const float ef = (float)r.e_fene;
                 et[S_FENE] += ef;
                 et[S_BEXC] += r.e_bexc;
                 et[S_STACK] += r.e_stack;
-                Eh += 0.5f * (ef + r.e_bexc + r.e_stack);
+                rec[6] = 0.5f * (ef + r.e_bexc + r.e_stack);
                 if (WANT_V) {
 #pragma unroll
-                    for (int s = 0; s < 6; ++s) vir[s] += r.vir[s];
+                    for (int q = 0; q < 6; ++q) rec[RV + q] = r.vir[q];
+                }
+                if (WANT_E) { rec[RS + 0] = ef; rec[RS + 1] = r.e_bexc; rec[RS + 2] = r.e_stack; }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NREC; ++q) s_rec[q][t] = rec[q];
+        __syncthreads();
+        // this particle's records of the round: near entries, bond slots, far entries
+        if (active) {
+            const uint32_t r1 = r0 + 256;
+            uint32_t lo[3] = {s_off[0][slot], B0 + 2 * slot, F0 + s_off[1][slot]};
+            uint32_t hi[3] = {s_off[0][slot + 1], NB ? B0 + 2 * slot + 2 : B0 + 2 * slot, F0 + s_off[1][slot + 1]};
+#pragma unroll
+            for (int g = 0; g < 3; ++g) {
+                const uint32_t x0 = max(lo[g], r0), x1 = min(hi[g], r1);
+                for (uint32_t x = x0 + lane; x < x1; x += TPP) {
+#pragma unroll
+                    for (int q = 0; q < NREC; ++q) acc[q] += s_rec[q][x - r0];
                 }
             }
         }
+        __syncthreads();
     }
     if (COUNT) {
 #pragma unroll
-        for (int s = 0; s < 14; ++s)
-            if (cnt[s]) atomicAdd(&a.counts[s], (unsigned long long)cnt[s]);
+        for (int k = 0; k < 14; ++k)
+            if (cnt[k]) atomicAdd(&a.counts[k], (unsigned long long)cnt[k]);
     }
     // reduce the TPP lanes of a particle (adjacent lanes of one wavefront)
 #pragma unroll
-    for (int m = TPP / 2; m > 0; m >>= 1) {
-        F.x += __shfl_xor(F.x, m); F.y += __shfl_xor(F.y, m); F.z += __shfl_xor(F.z, m);
-        T.x += __shfl_xor(T.x, m); T.y += __shfl_xor(T.y, m); T.z += __shfl_xor(T.z, m);
-        Eh += __shfl_xor(Eh, m);
-        if (WANT_V) {
+    for (uint32_t m = TPP / 2; m > 0; m >>= 1) {
 #pragma unroll
-            for (int s = 0; s < 6; ++s) vir[s] += __shfl_xor(vir[s], m);
-        }
+        for (int q = 0; q < NREC; ++q) acc[q] += __shfl_xor(acc[q], m);
     }
-    if (WANT_E && a.split) {
-        float s0 = et[S_FENE], s1 = et[S_BEXC], s2 = et[S_STACK];
-#pragma unroll
-        for (int m = TPP / 2; m > 0; m >>= 1) {
-            s0 += __shfl_xor(s0, m); s1 += __shfl_xor(s1, m); s2 += __shfl_xor(s2, m);
-        }
-        if (active && lane == 0) a.split[i] = make_double4(0.5 * s0, 0.5 * s1, 0.5 * s2, 0.0);
-    }
+    if (WANT_E && a.split && active && lane == 0)
+        a.split[i] = make_double4(0.5 * acc[RS + 0], 0.5 * acc[RS + 1], 0.5 * acc[RS + 2], 0.0);
     if (active && lane == 0) {
-        float4 fo = make_float4(F.x, F.y, F.z, Eh);
-        float4 to = make_float4(T.x, T.y, T.z, 0.f);
+        float4 fo = make_float4(acc[0], acc[1], acc[2], acc[6]);
+        float4 to = make_float4(acc[3], acc[4], acc[5], 0.f);
         if (FUSED) {
             // Langevin second half of step k with the force just computed, merged with the
             // first half of step k + 1: both touch only particle i's own state, and the
@@ -420,18 +519,18 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
             merged_particle(a.st, i, pi, e_v, e_q, e_p, fo, to, rebuild ? pi : e_plast, charge_i);
         } else {
             if (a.accumulate) {
-                float4 f = a.force[i], t = a.torque[i];
+                float4 f = a.force[i], tq = a.torque[i];
                 fo.x += f.x; fo.y += f.y; fo.z += f.z; fo.w += f.w;
-                to.x += t.x; to.y += t.y; to.z += t.z;
+                to.x += tq.x; to.y += tq.y; to.z += tq.z;
             }
             a.force[i] = fo;
             a.torque[i] = to;
             if (WANT_V) {
 #pragma unroll
-                for (int s = 0; s < 6; ++s) {
-                    float v = vir[s];
-                    if (a.accumulate) v += a.virial[s * a.vpitch + i];
-                    a.virial[s * a.vpitch + i] = v;
+                for (int q = 0; q < 6; ++q) {
+                    float v = acc[RV + q];
+                    if (a.accumulate) v += a.virial[q * a.vpitch + i];
+                    a.virial[q * a.vpitch + i] = v;
                 }
             }
             if (a.integrate) {
@@ -444,24 +543,24 @@ __global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
             }
         }
     }
-    if (FUSED && tid == 0) *a.st.step_out = *a.st.step_in + 1ull;
-    if (a.rebuild_clear && tid == 0) *a.rebuild_clear = 0u;
+    if (FUSED && t == 0 && xcd_block() == 0) *a.st.step_out = *a.st.step_in + 1ull;
+    if (a.rebuild_clear && t == 0 && xcd_block() == 0) *a.rebuild_clear = 0u;
     if (WANT_E) {
         // per-term totals: FP64 from here on (wave shuffle, then across waves in LDS)
         __shared__ double red[256 / 64][8];
-        const int wid = threadIdx.x / 64, wl = threadIdx.x % 64;
+        const int wid = t / 64, wl = t % 64;
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            double v = (double)et[s];
+        for (int q = 0; q < 8; ++q) {
+            double v = (double)et[q];
 #pragma unroll
             for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
-            if (wl == 0) red[wid][s] = v;
+            if (wl == 0) red[wid][q] = v;
         }
         __syncthreads();
-        if (threadIdx.x < 8) {
+        if (t < 8) {
             double v = 0.0;
-            for (int w = 0; w < (int)(blockDim.x / 64); ++w) v += red[w][threadIdx.x];
-            a.epart[(size_t)blockIdx.x * 8 + threadIdx.x] = 0.5 * v;  // full list: each pair seen twice
+            for (int w = 0; w < (int)(blockDim.x / 64); ++w) v += red[w][t];
+            a.epart[(size_t)blockIdx.x * 8 + t] = 0.5 * v;  // full list: each pair seen twice
         }
     }
 }
@@ -521,7 +620,7 @@ template<class T> struct DBuf {
 
 constexpr int kBlock = 256;
 constexpr int kTPPMax = 32;
-constexpr size_t kMaxLds = 48 * 1024;
+constexpr size_t kMaxLds = 40 * 1024;  // rebuild rows; the kernel's own LDS (records, particle data) is up to 23 KiB
 constexpr uint32_t kMaxNeigh = 768;  // 8 particles x 2 rows x 768 x 4 B = 48 KiB of LDS at 32 lanes
 
 }  // namespace
@@ -539,7 +638,7 @@ struct oxdna_ctx {
     DBuf<float4> fr0, fr1, fr2, force, torque;
     DBuf<float> virial;
     DBuf<int> bpart, binfo;
-    DBuf<uint32_t> n_neigh, nlist, cell_count, cell_members;
+    DBuf<uint32_t> n_neigh, n_near, nlist, cell_count, cell_members;
     DBuf<size_t> head;
     DBuf<double> epart, energy;
     DBuf<DevFlags> flags;
@@ -592,7 +691,7 @@ struct oxdna_ctx {
         pos.release(); vel.release(); quat.release(); angmom.release(); pos_last.release();
         bd.release(); bd_tq.release();
         charge.release(); fr0.release(); fr1.release(); fr2.release(); force.release(); torque.release();
-        virial.release(); bpart.release(); binfo.release(); n_neigh.release(); nlist.release();
+        virial.release(); bpart.release(); binfo.release(); n_neigh.release(); n_near.release(); nlist.release();
         cell_count.release(); cell_members.release(); head.release(); epart.release(); energy.release();
         flags.release(); dparams.release();
         rep_off.release(); rep_order.release(); rep_box.release(); rep_energy.release(); rep_bias.release();
@@ -804,10 +903,12 @@ void check_force_args(const ForceArgs& a);
 template<bool E, bool V, bool CNT, bool FUSED = false>
 void launch_k(int tpp, dim3 g, dim3 b, size_t lds, hipStream_t st, const ForceArgs& a) {
     check_force_args(a);
+    // tpp lanes per particle: 256 / tpp particles per workgroup
     switch (tpp) {
-    case 8: hipLaunchKernelGGL((k_forces<8, E, V, CNT, FUSED>), g, b, lds, st, a); break;
+    case 4: hipLaunchKernelGGL((k_forces<64, E, V, CNT, FUSED>), g, b, lds, st, a); break;
+    case 8: hipLaunchKernelGGL((k_forces<32, E, V, CNT, FUSED>), g, b, lds, st, a); break;
     case 16: hipLaunchKernelGGL((k_forces<16, E, V, CNT, FUSED>), g, b, lds, st, a); break;
-    default: hipLaunchKernelGGL((k_forces<32, E, V, CNT, FUSED>), g, b, lds, st, a); break;
+    default: hipLaunchKernelGGL((k_forces<8, E, V, CNT, FUSED>), g, b, lds, st, a); break;
     }
 }
 void check_force_args(const ForceArgs& a) {
@@ -940,6 +1041,7 @@ ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild, u
     a.pos = pos_buf(c, k);
     a.fr0 = fr_buf(c->fr0, c, k); a.fr1 = fr_buf(c->fr1, c, k); a.fr2 = fr_buf(c->fr2, c, k);
     a.n_neigh = c->n_neigh.p;
+    a.n_near = c->n_near.p;
     a.nlist = c->nlist.p;
     a.head = c->head.p;
     a.bpart = c->bpart.p;
@@ -1087,7 +1189,7 @@ StepArgs step_args(oxdna_ctx* c, bool with_cells, uint64_t t) {
 // particle epilogue would cost more issue slots than a kernel with a thread per particle.
 bool fuse_second_half(const oxdna_ctx* c) {
     const size_t waves = ((size_t)c->N * c->tpp + 63) / 64;
-    return waves <= 3072;  // resident at once: 3 waves per SIMD (the fused kernel's occupancy) on 256 CUs
+    return waves <= 2048;  // resident at once: 2 waves per SIMD (the fused kernel's occupancy) on 256 CUs
 }
 
 // A chunk of a run is: the first half of step 1 (k_step1), inner steps 1 ... n - 1 (the
@@ -1140,7 +1242,7 @@ void autotune(oxdna_ctx* c, uint32_t m) {
         return;
     }
     if (c->tuned_mask == m && c->tuned_n == c->N) return;
-    const int cand[3] = {8, 16, 32};
+    const int cand[4] = {4, 8, 16, 32};
     float best = 1e30f;
     int best_tpp = c->tpp;
     hipEvent_t e0, e1;
@@ -1508,7 +1610,8 @@ int oxdna_set_state(oxdna_ctx* c, unsigned int n, const double* pos, const doubl
             c->pos.release(); c->quat.release(); c->vel.release(); c->angmom.release(); c->pos_last.release();
             c->bd.release(); c->bd_tq.release();
             c->charge.release(); c->fr0.release(); c->fr1.release(); c->fr2.release(); c->force.release();
-            c->torque.release(); c->virial.release(); c->n_neigh.release(); c->nlist.release(); c->head.release();
+            c->torque.release(); c->virial.release(); c->n_neigh.release(); c->n_near.release(); c->nlist.release();
+            c->head.release();
             c->epart.release();
             c->have_bonds = false;
             c->n_rep = 0;  // the replica layout no longer fits
@@ -1518,6 +1621,7 @@ int oxdna_set_state(oxdna_ctx* c, unsigned int n, const double* pos, const doubl
         c->bd.alloc(n); c->bd_tq.alloc(n);
         c->charge.alloc(n); c->fr0.alloc(2 * (size_t)n); c->fr1.alloc(2 * (size_t)n); c->fr2.alloc(2 * (size_t)n);
         c->force.alloc(n); c->torque.alloc(n); c->virial.alloc(6 * (size_t)n); c->n_neigh.alloc(n);
+        c->n_near.alloc(n);
         c->epart.alloc((size_t)blocks_for((size_t)n * kTPPMax) * 8);
         HIP_OK(hipMemcpyAsync(c->pos.p, p.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->stream));
         // the second buffer too: a run reads its stale positions (to clear cells)
@@ -1967,8 +2071,8 @@ int oxdna_get_stats(oxdna_ctx* c, uint64_t* n_builds, uint32_t* max_neigh, uint6
 int oxdna_set_lanes_per_particle(oxdna_ctx* c, int lanes) {
     if (!c) return fail(nullptr, "NULL context");
     return guard(c, [&] {
-        if (lanes != 0 && lanes != 8 && lanes != 16 && lanes != 32)
-            throw Fail("oxdna_set_lanes_per_particle: 0 (autotune), 8, 16 or 32");
+        if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32)
+            throw Fail("oxdna_set_lanes_per_particle: 0 (autotune), 4, 8, 16 or 32");
         if (lanes != 0 && lds_bytes(c, lanes) > kMaxLds)
             throw Fail("oxdna_set_lanes_per_particle: neighbour rows too long for that many particles per block");
         c->fixed_tpp = lanes;
@@ -2070,7 +2174,7 @@ int oxdna_pair_forces(int term, float* d_force, float* d_torque, float* d_virial
         P.mask = 1u << term;
         const DParams* dP = stage_params(s, P, st);
         driver_frames(s, N, d_pos, d_orientation, d_charge, st);
-        const int tpp = 16;
+        const int tpp = 8;
         uint32_t blocks = blocks_for((size_t)N * tpp);
         s.epart.alloc((size_t)blocks * 8);
         ForceArgs a{};
@@ -2140,7 +2244,7 @@ int oxdna_bonded_forces(float* d_force, float* d_torque, float* d_virial, size_t
         a.n_neigh = s.zeros.p;
         a.nlist = s.zeros.p;
         a.head = s.zhead.p;
-        const int tpp = 16;
+        const int tpp = 8;
         s.epart.alloc((size_t)blocks_for((size_t)N * tpp) * 8);
         a.epart = s.epart.p;
         // split energies need the energy instantiation

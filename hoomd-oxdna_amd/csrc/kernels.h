@@ -221,6 +221,7 @@ struct ForceArgs {
     const float4* fr1;     // a2, type (int bits)
     const float4* fr2;     // a3, unused
     uint32_t* n_neigh;
+    uint32_t* n_near;      // leading class-0 entries of each row (NULL: rows without classes, all near)
     uint32_t* nlist;
     const size_t* head;
     const int* bpart;      // 2 per particle: partner or -1

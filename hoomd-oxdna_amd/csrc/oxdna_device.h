@@ -94,7 +94,7 @@ enum { MAX_BOND_TYPES = 4 };
 
 // All term parameters of one force field (uniform over type pairs apart from the
 // per-type-pair tables); read with scalar loads.
-struct DParams {
+struct alignas(16) DParams {  // a multiple of 16 bytes: kernels copy it to LDS in 16-byte words
     uint32_t mask;            // bits of oxdna_term present
     int ntypes;
     float rcutsq[6][16];      // per nonbonded term, per type pair

@@ -128,7 +128,7 @@ class Simulation:
     """One system on one GPU, driven through the engine's C ABI."""
 
     def __init__(self, device=None, seed=0, use_graph=True, lanes_per_particle=0):
-        """lanes_per_particle: 8, 16 or 32 lanes of the force kernel per particle, or 0
+        """lanes_per_particle: 4, 8, 16 or 32 lanes of the force kernel per particle, or 0
         to autotune (the choice fixes the summation order: pin it for bit-reproducible
         runs across processes)."""
         self.device = device if device is not None else GPU()

@@ -304,12 +304,13 @@ int oxdna_time_forces(oxdna_ctx* ctx, unsigned int term_mask, unsigned int nrep,
 int oxdna_count_evaluations(oxdna_ctx* ctx, unsigned int term_mask, uint64_t out[14]);
 /* Statistics: number of neighbour-list builds and the current list capacity. */
 int oxdna_get_stats(oxdna_ctx* ctx, uint64_t* n_builds, uint32_t* max_neigh, uint64_t* step);
-/* Lanes per particle of the force kernel: 8, 16 or 32, or 0 (default) to let the
- * autotuner time the three at the first run. The choice sets the order of each
- * particle's force sums, so reruns are bit-identical only with the same value: pin it
- * for bit-reproducible trajectories across processes. */
+/* Lanes per particle of the force kernel: 4, 8, 16 or 32 (a workgroup of 256 threads
+ * then owns 64, 32, 16 or 8 particles), or 0 (default) to let the autotuner time 4, 8
+ * and 16 at the first run. The choice sets the order of each particle's force sums, so
+ * reruns are bit-identical only with the same value: pin it for bit-reproducible
+ * trajectories across processes. */
 int oxdna_set_lanes_per_particle(oxdna_ctx* ctx, int lanes);
-/* Autotuner choices: lanes per particle of the force kernel (8, 16 or 32) and the
+/* Autotuner choices: lanes per particle of the force kernel (4, 8, 16 or 32) and the
  * workgroup size of the replica kernel (64..512; 0 before the first replica run). */
 int oxdna_get_tuning(oxdna_ctx* ctx, int* lanes_per_particle, int* replica_workgroup);
 
