@@ -55,13 +55,17 @@ def flops_per_launch(counts, census, evals_per_pair=1):
     return total
 
 
-def traffic_from_profile():
-    """HBM bytes per force launch from the committed rocprofv3 counter summary, if any."""
+def traffic_from_profile(kernel):
+    """HBM bytes per force launch from the committed rocprofv3 counter summary, when it
+    was taken on the same kernel instantiation (else None)."""
     path = os.path.join(ROOT, "profiles", "k_forces_oxdna2_8k_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh).get("hbm_bytes_per_launch")
+        prof = json.load(fh)
+    if not prof.get("kernel", "").startswith(kernel):
+        return None
+    return prof.get("hbm_bytes_per_launch")
 
 
 def cpu_baseline(system, forces, seconds=12.0):
@@ -213,7 +217,7 @@ def bench_large(model_name, steps, warmup, census_all):
         "steps": steps, "steps_per_s": round(steps / dt, 1), "ms_per_step": round(dt * 1e3 / steps, 4),
         "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 6), "traffic": None,
-                     "kernel": f"k_forces<{tpp.value}>", "kernel_ms": round(avg_ms.value, 4),
+                     "kernel": f"k_forces<{256 // tpp.value}>", "kernel_ms": round(avg_ms.value, 4),
                      "flops_per_launch": flops},
         "pair_evaluations_per_s": round(sum(counts[2 * t] for t in range(6)) / 2 * steps / dt, 1),
         "nlist_builds": st["nlist_builds"], "max_neigh": st["max_neigh"], "generate_s": round(t_gen, 1),
@@ -316,7 +320,8 @@ def main():
                    "n_nucleotides": system.n, "graph": not args.no_graph},
         "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 6),
-                     "traffic": traffic_from_profile(), "kernel": f"k_forces<{tpp.value}>",
+                     "traffic": traffic_from_profile(f"k_forces<{256 // tpp.value},"),
+                     "kernel": f"k_forces<{256 // tpp.value}>",
                      "kernel_ms": round(avg_ms.value, 5), "flops_per_launch": flops},
         "pair_evaluations_per_s": round(pair_evals * steps_per_s, 1),
         "nlist_builds": stats["nlist_builds"],
