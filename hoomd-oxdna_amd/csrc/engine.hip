@@ -620,7 +620,9 @@ template<class T> struct DBuf {
 
 constexpr int kBlock = 256;
 constexpr int kTPPMax = 32;
-constexpr size_t kMaxLds = 40 * 1024;  // rebuild rows; the kernel's own LDS (records, particle data) is up to 23 KiB
+// LDS of the rebuild rows; with the force kernel's own LDS (records, particle data,
+// parameters: up to 26 KiB) a workgroup stays within 64 KiB
+constexpr size_t kMaxLds = 36 * 1024;
 constexpr uint32_t kMaxNeigh = 768;  // 8 particles x 2 rows x 768 x 4 B = 48 KiB of LDS at 32 lanes
 
 }  // namespace

@@ -30,9 +30,9 @@ def _say(*a):
     sys.stdout.flush()
 
 
-def engine_eval(system, forces):
+def engine_eval(system, forces, lanes=0):
     """Attach forces to a Simulation, run(0), read per-force outputs."""
-    s = sim.Simulation(device=sim.GPU())
+    s = sim.Simulation(device=sim.GPU(), lanes_per_particle=lanes)
     s.create_state_from_snapshot(system.snapshot())
     s.operations.integrator = sim.Integrator(dt=0.003, forces=forces)
     s.run(0)
@@ -44,9 +44,9 @@ def engine_eval(system, forces):
     return s, out
 
 
-def check_against_oracle(system, forces, label):
+def check_against_oracle(system, forces, label, lanes=0):
     orc = O.from_system(system, forces)
-    _, got = engine_eval(system, forces)
+    _, got = engine_eval(system, forces, lanes)
     n = system.n
     worst = {}
     problems = []
@@ -118,6 +118,27 @@ def test_gas_of_duplexes_parity():
     system = H.jiggle(systems.duplex_gas(24, 10, box=18.0, seed=11, min_dist=1.2), seed=5, dx=0.03, dq=0.05)
     forces, _ = dna2.forces(fused=True)
     check_against_oracle(system, forces, "gas dna2 fused")
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16, 32])
+def test_particles_per_workgroup_variants(lanes):
+    """Every force-kernel layout (256 / lanes particles per workgroup: how the near pairs,
+    bond slots and far pairs of a workgroup pack onto wavefronts) against the oracle."""
+    system = H.jiggle(systems.duplex_gas(40, 12, box=30.0, seed=12, min_dist=1.2), seed=9, dx=0.03, dq=0.05)
+    forces, _ = dna2.forces(fused=True)
+    check_against_oracle(system, forces, f"gas dna2 fused, {lanes} lanes", lanes=lanes)
+
+
+def test_pinned_lanes_reproducible():
+    """Same lanes per particle, same seed: two runs give bit-identical trajectories."""
+    system = H.cfg1_system(2)
+    out = []
+    for _ in range(2):
+        forces, _ = dna2.forces(fused=True)
+        s = _langevin_sim(system, forces, seed=4, lanes=8)
+        s.run(200)
+        out.append(s.state.get_snapshot().particles.position.copy())
+    assert np.array_equal(out[0], out[1])
 
 
 def test_coaxial_stacking_nicked_helix():
