@@ -261,9 +261,11 @@ __device__ __forceinline__ uint32_t owner_of(const uint32_t* off, uint32_t x) {
 // FUSED: a run step whose epilogue also takes the particle through the Langevin second
 // half of step k and the first half of step k + 1 (a.integrate == 2); its loads are
 // issued before the pair loop, and the force and torque, consumed in the epilogue, are
-// not stored.
+// not stored. Register budget: the plain kernel at 3 waves per SIMD (168 VGPRs, no
+// spills: large systems are bound by issue rate across many waves); the fused one, used
+// only while the grid holds <= 2 waves per SIMD, unconstrained.
 template<int P, bool WANT_E, bool WANT_V, bool COUNT, bool FUSED = false>
-__global__ __launch_bounds__(256) void k_forces(ForceArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FUSED ? 1 : 3))) void k_forces(ForceArgs a) {
     constexpr uint32_t TPP = 256 / P;
     constexpr int NREC = kRecBase + (WANT_V ? 6 : 0) + (WANT_E ? 3 : 0);
     constexpr int RV = kRecBase, RS = kRecBase + (WANT_V ? 6 : 0);  // virial, split energies

@@ -102,13 +102,24 @@ __device__ __forceinline__ float pair_terms(const DParams& P, uint32_t mask, int
     if ((mask & (1u << T_HBONDCROSS)) && rsq < P.rcutsq[T_HBONDCROSS][tp]) {
         BB g;
         if (bb_geom(Rc, fi, fj, P.hbx.base, g)) {
-            float eps = P.hbx.hb_eps[tp], rv, rd;
-            f1_vd(g.rmod, P.hbx.f1, eps, eps * P.hbx.shift_factor, rv, rd);
-            float e1 = bb_term(g, fi, fj, rv, rd, P.hbx.hb_t1, P.hbx.hb_t23, P.hbx.hb_t4, P.hbx.hb_t78,
-                               false, true, Fp, T);
-            f2_vd(g.rmod, P.hbx.f2, rv, rd);
-            float e2 = bb_term(g, fi, fj, rv, rd, P.hbx.cs_t1, P.hbx.cs_t23, P.hbx.cs_t4, P.hbx.cs_t78,
-                               true, P.hbx.cs_t4_enabled != 0, Fp, T);
+            const float eps = P.hbx.hb_eps[tp];
+            float rv1, rd1, rv2, rd2;
+            f1_vd(g.rmod, P.hbx.f1, eps, eps * P.hbx.shift_factor, rv1, rd1);
+            f2_vd(g.rmod, P.hbx.f2, rv2, rd2);
+            const bool t4cs = P.hbx.cs_t4_enabled != 0;
+            const bool on1 = bb_active(g, rv1, P.hbx.hb_t1, P.hbx.hb_t23, P.hbx.hb_t4, P.hbx.hb_t78, false, true);
+            const bool on2 = bb_active(g, rv2, P.hbx.cs_t1, P.hbx.cs_t23, P.hbx.cs_t4, P.hbx.cs_t78, true, t4cs);
+            float e1 = 0.f, e2 = 0.f;
+            if (on1 || on2) {
+                BBAng an;  // one arccos per cosine for both terms
+                bb_angles(g, an);
+                if (on1)
+                    e1 = bb_term_ang(g, an, fi, fj, rv1, rd1, P.hbx.hb_t1, P.hbx.hb_t23, P.hbx.hb_t4, P.hbx.hb_t78,
+                                     false, true, Fp, T);
+                if (on2)
+                    e2 = bb_term_ang(g, an, fi, fj, rv2, rd2, P.hbx.cs_t1, P.hbx.cs_t23, P.hbx.cs_t4, P.hbx.cs_t78,
+                                     true, t4cs, Fp, T);
+            }
             et[S_HB] += e1; et[S_CR] += e2; ep += e1 + e2;
             if (COUNT) cnt[2 * T_HBONDCROSS + 1] += (e1 != 0.f || e2 != 0.f);
         }

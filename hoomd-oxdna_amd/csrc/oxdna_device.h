@@ -150,21 +150,21 @@ __device__ __forceinline__ void f2_vd(float r, const F2P& s, float& v, float& d)
 }
 // f4 of the angle th and its torque coefficient (d f4/d th)/sin(th), given sin(th)
 // (reference src/OxDNANumerics.h:201-244, inner-branch 1e-8 guard included).
+// Branch-free: both pieces are computed and selected (a wavefront's lanes would run
+// both branches anyway).
 __device__ __forceinline__ void f4_vd(float th, float sth, const F4P& p, float& v, float& dsin) {
-    float t = th - p.t0;
-    float m = 1.f;
-    if (t < 0.f) { t = -t; m = -1.f; }
-    v = 0.f; dsin = 0.f;
-    if (t < p.tc) {
-        if (t > p.ts) {
-            float e = p.tc - t;
-            v = p.b * e * e;
-            dsin = -m * 2.f * p.b * e / sth;
-        } else {
-            v = 1.f - p.a * t * t;
-            dsin = (sth * sth > 1e-8f) ? -m * 2.f * p.a * t / sth : -m * 2.f * p.a;
-        }
-    }
+    const float t0 = th - p.t0;
+    const float t = fabsf(t0);
+    const float m = t0 < 0.f ? -1.f : 1.f;
+    const float e = p.tc - t;
+    const float inv = 1.f / sth;
+    const bool tail = t > p.ts;
+    const float v_tail = p.b * e * e, d_tail = -m * 2.f * p.b * e * inv;
+    const float v_in = 1.f - p.a * t * t;
+    const float d_in = (sth * sth > 1e-8f) ? -m * 2.f * p.a * t * inv : -m * 2.f * p.a;
+    const bool in = t < p.tc;
+    v = in ? (tail ? v_tail : v_in) : 0.f;
+    dsin = in ? (tail ? d_tail : d_in) : 0.f;
 }
 __device__ __forceinline__ float f4_v(float th, const F4P& p) {
     float t = fabsf(th - p.t0);
@@ -266,22 +266,48 @@ __device__ __forceinline__ bool bb_geom(V Rc, const Frame& fi, const Frame& fj, 
     return true;
 }
 
-// One BASE-BASE term given its radial value / derivative; adds force and torque on i
-// (reference src/OxDNABaseBase.h:128-219, the p = i half).
-__device__ __forceinline__ float bb_term(const BB& g, const Frame& fi, const Frame& fj, float rv, float rd,
-                                         const F4P& t1, const F4P& t23, const F4P& t4, const F4P& t78,
-                                         bool sym, bool use_t4, V& F, V& T) {
-    if (rv == 0.f) return 0.f;  // zero radial => zero energy, no force (early exit of the reference)
-    // a zero angular factor makes the term zero: screen all six in cosine space first
-    if (!(f4_nz(g.c1, t1, false) && f4_nz(g.c2, t23, false) && f4_nz(g.c3, t23, false) &&
-          (!use_t4 || f4_nz(g.c4, t4, sym)) && f4_nz(g.c7, t78, sym) && f4_nz(g.c8, t78, sym)))
-        return 0.f;
-    Ang A1 = angular(g.c1, t1, false, -1.f);
-    Ang A2 = angular(g.c2, t23, false, -1.f);
-    Ang A3 = angular(g.c3, t23, false, 1.f);
-    Ang A4 = use_t4 ? angular(g.c4, t4, sym, 1.f) : Ang{1.f, 0.f};
-    Ang A7 = angular(g.c7, t78, sym, -1.f);
-    Ang A8 = angular(g.c8, t78, sym, 1.f);
+// The angles of the six BASE-BASE cosines (c1, c2, c3, c4, c7, c8) and their sines: one
+// arccos each, shared by the two terms of the fused hydrogen-bonding + cross-stacking
+// evaluator, which read the same geometry.
+struct BBAng { float th[6], s[6]; };
+__device__ __forceinline__ void bb_angles(const BB& g, BBAng& a) {
+    const float c[6] = {g.c1, g.c2, g.c3, g.c4, g.c7, g.c8};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a.th[k] = acos_as(clampc(c[k]), a.s[k]);
+}
+// Angular factor from a precomputed angle (angular() without the arccos).
+__device__ __forceinline__ Ang angular_th(float th, float s, const F4P& p, bool sym, float sign) {
+    Ang a;
+    f4_vd(th, s, p, a.v, a.d);
+    if (sym) {
+        float v2, d2;
+        f4_vd(3.14159265358979f - th, s, p, v2, d2);
+        a.v += v2;
+        a.d -= d2;
+    }
+    a.d *= sign;
+    return a;
+}
+
+// Whether a BASE-BASE term can be nonzero: nonzero radial (the reference's early exit)
+// and all six angular factors possibly nonzero, screened in cosine space.
+__device__ __forceinline__ bool bb_active(const BB& g, float rv, const F4P& t1, const F4P& t23, const F4P& t4,
+                                          const F4P& t78, bool sym, bool use_t4) {
+    return rv != 0.f && f4_nz(g.c1, t1, false) && f4_nz(g.c2, t23, false) && f4_nz(g.c3, t23, false) &&
+           (!use_t4 || f4_nz(g.c4, t4, sym)) && f4_nz(g.c7, t78, sym) && f4_nz(g.c8, t78, sym);
+}
+
+// One BASE-BASE term given its radial value / derivative and the angles; adds force and
+// torque on i (reference src/OxDNABaseBase.h:128-219, the p = i half).
+__device__ __forceinline__ float bb_term_ang(const BB& g, const BBAng& an, const Frame& fi, const Frame& fj, float rv,
+                                             float rd, const F4P& t1, const F4P& t23, const F4P& t4,
+                                             const F4P& t78, bool sym, bool use_t4, V& F, V& T) {
+    Ang A1 = angular_th(an.th[0], an.s[0], t1, false, -1.f);
+    Ang A2 = angular_th(an.th[1], an.s[1], t23, false, -1.f);
+    Ang A3 = angular_th(an.th[2], an.s[2], t23, false, 1.f);
+    Ang A4 = use_t4 ? angular_th(an.th[3], an.s[3], t4, sym, 1.f) : Ang{1.f, 0.f};
+    Ang A7 = angular_th(an.th[4], an.s[4], t78, sym, -1.f);
+    Ang A8 = angular_th(an.th[5], an.s[5], t78, sym, 1.f);
     float p12 = A1.v * A2.v, p34 = A3.v * A4.v, p78 = A7.v * A8.v;
     float prod = p12 * p34 * p78;
     float e = rv * prod;
@@ -306,6 +332,17 @@ __device__ __forceinline__ float bb_term(const BB& g, const Frame& fi, const Fra
     F -= fq;
     T += tp;
     return e;
+}
+
+
+// One BASE-BASE term (screen, angles, evaluation).
+__device__ __forceinline__ float bb_term(const BB& g, const Frame& fi, const Frame& fj, float rv, float rd,
+                                         const F4P& t1, const F4P& t23, const F4P& t4, const F4P& t78,
+                                         bool sym, bool use_t4, V& F, V& T) {
+    if (!bb_active(g, rv, t1, t23, t4, t78, sym, use_t4)) return 0.f;
+    BBAng an;
+    bb_angles(g, an);
+    return bb_term_ang(g, an, fi, fj, rv, rd, t1, t23, t4, t78, sym, use_t4, F, T);
 }
 
 // Coaxial stacking (reference src/EvaluatorPairOxDNACoaxStacking.h:134-372), p = i half.
