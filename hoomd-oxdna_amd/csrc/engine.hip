@@ -263,9 +263,11 @@ __device__ __forceinline__ uint32_t owner_of(const uint32_t* off, uint32_t x) {
 // issued before the pair loop, and the force and torque, consumed in the epilogue, are
 // not stored. Register budget: the plain kernel at 3 waves per SIMD (168 VGPRs, no
 // spills: large systems are bound by issue rate across many waves); the fused one, used
-// only while the grid holds <= 2 waves per SIMD, unconstrained.
+// only while the grid holds <= 2 waves per SIMD, and the single-point energy + virial
+// one, unconstrained.
 template<int P, bool WANT_E, bool WANT_V, bool COUNT, bool FUSED = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FUSED ? 1 : 3))) void k_forces(ForceArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((FUSED || (WANT_E && WANT_V)) ? 1 : 3)))
+void k_forces(ForceArgs a) {
     constexpr uint32_t TPP = 256 / P;
     constexpr int NREC = kRecBase + (WANT_V ? 6 : 0) + (WANT_E ? 3 : 0);
     constexpr int RV = kRecBase, RS = kRecBase + (WANT_V ? 6 : 0);  // virial, split energies
