@@ -113,12 +113,21 @@ __device__ __forceinline__ float pair_terms(const DParams& P, uint32_t mask, int
             if (on1 || on2) {
                 BBAng an;  // one arccos per cosine for both terms
                 bb_angles(g, an);
-                if (on1)
-                    e1 = bb_term_ang(g, an, fi, fj, rv1, rd1, P.hbx.hb_t1, P.hbx.hb_t23, P.hbx.hb_t4, P.hbx.hb_t78,
-                                     false, true, Fp, T);
-                if (on2)
-                    e2 = bb_term_ang(g, an, fi, fj, rv2, rd2, P.hbx.cs_t1, P.hbx.cs_t23, P.hbx.cs_t4, P.hbx.cs_t78,
-                                     true, t4cs, Fp, T);
+                BBW w;  // both terms' weights, assembled once
+                bbw_zero(w);
+                if (on1) {
+                    const BBW w1 = bb_weights(g, an, rv1, rd1, P.hbx.hb_t1, P.hbx.hb_t23, P.hbx.hb_t4, P.hbx.hb_t78,
+                                              false, true);
+                    e1 = w1.e;
+                    bbw_add(w, w1);
+                }
+                if (on2) {
+                    const BBW w2 = bb_weights(g, an, rv2, rd2, P.hbx.cs_t1, P.hbx.cs_t23, P.hbx.cs_t4, P.hbx.cs_t78,
+                                              true, t4cs);
+                    e2 = w2.e;
+                    bbw_add(w, w2);
+                }
+                if (e1 != 0.f || e2 != 0.f) bb_assemble(g, fi, fj, w, Fp, T);
             }
             et[S_HB] += e1; et[S_CR] += e2; ep += e1 + e2;
             if (COUNT) cnt[2 * T_HBONDCROSS + 1] += (e1 != 0.f || e2 != 0.f);

@@ -297,11 +297,18 @@ __device__ __forceinline__ bool bb_active(const BB& g, float rv, const F4P& t1, 
            (!use_t4 || f4_nz(g.c4, t4, sym)) && f4_nz(g.c7, t78, sym) && f4_nz(g.c8, t78, sym);
 }
 
-// One BASE-BASE term given its radial value / derivative and the angles; adds force and
-// torque on i (reference src/OxDNABaseBase.h:128-219, the p = i half).
-__device__ __forceinline__ float bb_term_ang(const BB& g, const BBAng& an, const Frame& fi, const Frame& fj, float rv,
-                                             float rd, const F4P& t1, const F4P& t23, const F4P& t4,
-                                             const F4P& t78, bool sym, bool use_t4, V& F, V& T) {
+// A BASE-BASE term's energy and the scalar weights of its force and torque: the
+// assembly (bb_assemble) is linear in them over vectors that depend only on the shared
+// geometry, so the two terms of the fused evaluator add their weights and assemble once.
+struct BBW { float e, wr, w1, w2, w3, w4, w7, w8; };
+__device__ __forceinline__ void bbw_zero(BBW& w) { w = BBW{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}; }
+__device__ __forceinline__ void bbw_add(BBW& w, const BBW& x) {
+    w.e += x.e; w.wr += x.wr; w.w1 += x.w1; w.w2 += x.w2; w.w3 += x.w3; w.w4 += x.w4; w.w7 += x.w7; w.w8 += x.w8;
+}
+// Weights of one term given its radial value / derivative and the angles (reference
+// src/OxDNABaseBase.h:128-219, the p = i half); zero weights when a factor vanishes.
+__device__ __forceinline__ BBW bb_weights(const BB& g, const BBAng& an, float rv, float rd, const F4P& t1,
+                                          const F4P& t23, const F4P& t4, const F4P& t78, bool sym, bool use_t4) {
     Ang A1 = angular_th(an.th[0], an.s[0], t1, false, -1.f);
     Ang A2 = angular_th(an.th[1], an.s[1], t23, false, -1.f);
     Ang A3 = angular_th(an.th[2], an.s[2], t23, false, 1.f);
@@ -310,30 +317,45 @@ __device__ __forceinline__ float bb_term_ang(const BB& g, const BBAng& an, const
     Ang A8 = angular_th(an.th[5], an.s[5], t78, sym, 1.f);
     float p12 = A1.v * A2.v, p34 = A3.v * A4.v, p78 = A7.v * A8.v;
     float prod = p12 * p34 * p78;
-    float e = rv * prod;
-    if (e == 0.f) return 0.f;
+    BBW w;
+    w.e = rv * prod;
+    if (w.e == 0.f) { bbw_zero(w); return w; }
     // products with one factor replaced by its coefficient
-    float w1 = rv * A1.d * A2.v * p34 * p78;
-    float w2 = rv * A1.v * A2.d * p34 * p78;
-    float w3 = rv * p12 * A3.d * A4.v * p78;
-    float w4 = rv * p12 * A3.v * A4.d * p78;
-    float w7 = rv * p12 * p34 * A7.d * A8.v;
-    float w8 = rv * p12 * p34 * A7.v * A8.d;
-    float inv = 1.f / g.rmod;
-    V fq = g.rdir * (-rd * prod);
-    fq += (fj.a1 + g.rdir * g.c2) * (w2 * inv);
-    fq += (fi.a1 - g.rdir * g.c3) * (w3 * inv);
-    fq += (fj.a3 + g.rdir * g.c7) * (w7 * inv);
-    fq += (fi.a3 - g.rdir * g.c8) * (w8 * inv);
-    V tp = cross(fi.a3, fj.a3) * w4 + cross(fi.a1, fj.a1) * w1;
-    tp -= cross(g.rdir, fi.a1) * w3;
-    tp -= cross(g.rdir, fi.a3) * w8;
+    const float inv = 1.f / g.rmod;
+    w.wr = -rd * prod;
+    w.w1 = rv * A1.d * A2.v * p34 * p78;
+    w.w2 = rv * A1.v * A2.d * p34 * p78 * inv;
+    w.w3 = rv * p12 * A3.d * A4.v * p78 * inv;
+    w.w4 = rv * p12 * A3.v * A4.d * p78;
+    w.w7 = rv * p12 * p34 * A7.d * A8.v * inv;
+    w.w8 = rv * p12 * p34 * A7.v * A8.d * inv;
+    return w;
+}
+// Force and torque on i from summed weights (w3, w8 already divided by |r| for the
+// force; the torque terms take them times |r|).
+__device__ __forceinline__ void bb_assemble(const BB& g, const Frame& fi, const Frame& fj, const BBW& w, V& F, V& T) {
+    V fq = g.rdir * w.wr;
+    fq += (fj.a1 + g.rdir * g.c2) * w.w2;
+    fq += (fi.a1 - g.rdir * g.c3) * w.w3;
+    fq += (fj.a3 + g.rdir * g.c7) * w.w7;
+    fq += (fi.a3 - g.rdir * g.c8) * w.w8;
+    V tp = cross(fi.a3, fj.a3) * w.w4 + cross(fi.a1, fj.a1) * w.w1;
+    tp -= cross(g.rdir, fi.a1) * (w.w3 * g.rmod);
+    tp -= cross(g.rdir, fi.a3) * (w.w8 * g.rmod);
     tp -= cross(g.base_i, fq);
     F -= fq;
     T += tp;
-    return e;
 }
-
+// One BASE-BASE term given its radial value / derivative and the angles; adds force and
+// torque on i.
+__device__ __forceinline__ float bb_term_ang(const BB& g, const BBAng& an, const Frame& fi, const Frame& fj, float rv,
+                                             float rd, const F4P& t1, const F4P& t23, const F4P& t4,
+                                             const F4P& t78, bool sym, bool use_t4, V& F, V& T) {
+    const BBW w = bb_weights(g, an, rv, rd, t1, t23, t4, t78, sym, use_t4);
+    if (w.e == 0.f) return 0.f;
+    bb_assemble(g, fi, fj, w, F, T);
+    return w.e;
+}
 
 // One BASE-BASE term (screen, angles, evaluation).
 __device__ __forceinline__ float bb_term(const BB& g, const Frame& fi, const Frame& fj, float rv, float rd,
