@@ -263,8 +263,11 @@ int oxdna_get_nlist_pairs(oxdna_ctx* ctx, int32_t* pairs, size_t max_pairs, size
  * gamma_r are per-particle-type drag coefficients (reference defaults 1.0). */
 int oxdna_set_langevin(oxdna_ctx* ctx, double dt, double kT, double gamma, double gamma_r,
                        uint64_t seed);
-/* Advances n steps, each step one fused force evaluation of term_mask. use_graph != 0
- * replays the step as a captured HIP graph. */
+/* Advances n Langevin steps (oxdna_set_langevin), each one fused force evaluation of
+ * term_mask with the integrator in the same launch on systems up to ~2000 wavefronts
+ * (a second kernel on larger ones). use_graph != 0 replays the steps 96 at a time as a
+ * captured HIP graph; graph and direct launches give bit-identical trajectories. The
+ * state (positions, orientations, momenta, forces) is complete when the call returns. */
 int oxdna_run(oxdna_ctx* ctx, unsigned int nsteps, unsigned int term_mask, int use_graph);
 int oxdna_get_state(oxdna_ctx* ctx, double* pos, double* quat, double* vel, double* angmom);
 /* Batched replicas: the particles given to oxdna_set_state are n_replicas
