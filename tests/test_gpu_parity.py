@@ -226,6 +226,43 @@ def test_graph_and_direct_steps_agree():
     assert np.array_equal(out[0], out[1])
 
 
+def test_run_grows_rows_mid_chunk():
+    """A run whose neighbour rows outgrow their capacity: a lattice of free nucleotides
+    (no bonds) imploding toward the box centre doubles its central density within the
+    run. The chunk is restored, the rows grow and it runs again; the state stays finite,
+    the row capacity grew, and the forces at the final state match the oracle."""
+    rng = np.random.default_rng(21)
+    m, a = 12, 1.8
+    L = m * a
+    g = (np.arange(m) - 0.5 * (m - 1)) * a
+    pos = np.array([[x, y, z] for x in g for y in g for z in g])
+    n = len(pos)
+    quat = rng.normal(size=(n, 4))
+    quat /= np.linalg.norm(quat, axis=1)[:, None]
+    typeid = rng.integers(0, 4, size=n)
+    none = -np.ones(n, dtype=int)
+    system = systems.System(pos, quat, typeid, np.zeros((0, 2), dtype=int), [L] * 3, np.arange(n),
+                            ["ACGT"[t] for t in typeid], none, none)
+    snap = system.snapshot()
+    snap.particles.velocity[:] = -0.25 * pos  # uniform compression: neighbours close in slowly
+    forces, _ = dna2.forces(fused=True)
+    s = sim.Simulation(device=sim.GPU(), seed=3)
+    s.create_state_from_snapshot(snap)
+    s.operations.integrator = sim.Integrator(dt=0.003, forces=forces,
+                                             methods=[sim.Langevin(kT=dna1.kT_from_temperature(296.15))])
+    s.run(0)
+    cap0 = s.stats()["max_neigh"]
+    s.run(600)
+    st = s.stats()
+    final = s.state.get_snapshot()
+    _say(f"implosion: row capacity {cap0} -> {st['max_neigh']}, {st['nlist_builds']} builds")
+    assert np.isfinite(final.particles.position).all()
+    assert st["max_neigh"] > cap0
+    after = systems.System(final.particles.position, final.particles.orientation, typeid, system.bonds, system.box,
+                           system.strand, system.bases, none, none)
+    check_against_oracle(after, dna2.forces(fused=True)[0], "after implosion")
+
+
 def _rot_ke(snap):
     """Rotational kinetic energy per particle, I = 1: L_body = 1/2 (conj(q) p).v."""
     q, p = snap.particles.orientation, snap.particles.angmom
