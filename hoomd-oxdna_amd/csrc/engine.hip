@@ -677,7 +677,6 @@ struct oxdna_ctx {
     uint32_t forces_mask = 0xffffffffu;  // mask of the forces currently in `force`
     hipGraphExec_t graph = nullptr;
     uint32_t graph_mask = 0;
-    int graph_steps = 0;
     uint64_t host_step = 0;
 
     // batched replicas (n_rep > 0): particles [rep_off[r], rep_off[r+1]) form replica r
@@ -707,7 +706,6 @@ struct oxdna_ctx {
     void drop_graph() {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
-        graph_steps = 0;
     }
 };
 
@@ -1232,7 +1230,6 @@ void build_graph(oxdna_ctx* c, uint32_t mask) {
     HIP_OK(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
     HIP_OK(hipGraphDestroy(g));
     c->graph_mask = mask;
-    c->graph_steps = kGraphSteps;
 }
 
 // Autotuner: lanes per particle (8, 16, 32) of the force kernel, timed at the
