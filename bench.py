@@ -254,7 +254,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the replica / "
-              "domain sharding across GPUs is described in DESIGN.md and not launched here.",
+              "domain sharding across GPUs is not launched here.",
               file=sys.stderr)
         sys.exit(2)
 
