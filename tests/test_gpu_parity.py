@@ -226,6 +226,46 @@ def test_graph_and_direct_steps_agree():
     assert np.array_equal(out[0], out[1])
 
 
+def _tiny(pos, quat, typeid, bonds, strand, box=20.0):
+    n = len(pos)
+    n3, n5 = -np.ones(n, dtype=int), -np.ones(n, dtype=int)
+    for a_, b_ in bonds:
+        n5[a_], n3[b_] = b_, a_
+    return systems.System(np.asarray(pos, float), np.asarray(quat, float), np.asarray(typeid), np.asarray(bonds),
+                          [box] * 3, np.asarray(strand), ["ACGT"[t] for t in typeid], n3, n5)
+
+
+def test_tiny_systems():
+    """Edge cases: one nucleotide (all forces and energies zero; a run moves it and stays
+    finite), a bonded dimer (bonded terms against the oracle; the bond partners are no
+    nonbonded pair) and two free nucleotides beyond every cut-off (all zero)."""
+    q0 = [1.0, 0.0, 0.0, 0.0]
+    one = _tiny([[0.0, 0.0, 0.0]], [q0], [0], np.zeros((0, 2), int), [0])
+    forces, _ = dna2.forces(fused=True)
+    s, got = engine_eval(one, forces)
+    for t, g in got.items():
+        assert np.all(g["E"] == 0.0) and np.all(g["F"] == 0.0) and np.all(g["T"] == 0.0), t
+    forces, _ = dna2.forces(fused=True)
+    s = _langevin_sim(one, forces, seed=2)
+    s.run(50)
+    p = s.state.get_snapshot().particles.position
+    assert np.isfinite(p).all() and np.abs(p).max() > 0.0
+
+    ang = 2 * np.pi / 10.34
+    dimer = _tiny([[0.0, 0.0, 0.0], [0.0, 0.0, 0.39]],
+                  [q0, [np.cos(ang / 2), 0.0, 0.0, np.sin(ang / 2)]], [0, 2], [[0, 1]], [0, 0])
+    for model in (dna1, dna2, rna2):  # (the oxRNA2 backbone is stretched past its FENE limit here:
+        # the reference's broken-bond energy, 1e12 per bond, and no force)
+        check_against_oracle(H.jiggle(dimer, seed=3, dx=0.01, dq=0.02), model.forces(fused=True)[0],
+                             f"dimer {model.__name__.split('.')[-1]}")
+
+    apart = _tiny([[-6.0, 0.0, 0.0], [6.0, 0.0, 0.0]], [q0, q0], [0, 3], np.zeros((0, 2), int), [0, 1])
+    forces, _ = dna2.forces(fused=True)
+    _, got = engine_eval(apart, forces)
+    for t, g in got.items():
+        assert np.all(g["F"] == 0.0) and np.all(g["E"] == 0.0), t
+
+
 def test_run_grows_rows_mid_chunk():
     """A run whose neighbour rows outgrow their capacity: a lattice of free nucleotides
     (no bonds) imploding toward the box centre doubles its central density within the
