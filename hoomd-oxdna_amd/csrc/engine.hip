@@ -840,6 +840,10 @@ void load_pair_params(DParams& P, int term, int nt, const void* params) {
         d.t1_mode = t[0].t1_mode; d.phi3_enabled = t[0].phi3_enabled; d.rna_coax = t[0].rna_coax;
         d.t1_sa = (float)t[0].t1_sa; d.t1_sb = (float)t[0].t1_sb;
         if (d.t1_mode != 0 && d.t1_mode != 1) throw Fail("coaxial stacking: t1_mode must be 0 or 1");
+        // theta1's second piece: mode 0 f4(2 pi - theta1), nonzero for theta1 > 2 pi - t0 - tc
+        // (theta1 <= pi); mode 1 SA (theta1 - SB)^2, nonzero for theta1 > SB
+        const double t1_lo = d.t1_mode == 0 ? 2.0 * M_PI - t[0].t1[2] - t[0].t1[4] : t[0].t1_sb;  // (a, b, t0, ts, tc)
+        d.t1_cx = t1_lo >= M_PI ? -2.0f : (t1_lo <= 0.0 ? 2.0f : (float)std::cos(t1_lo));
         break;
     }
     case OXDNA_TERM_DEBYE: {

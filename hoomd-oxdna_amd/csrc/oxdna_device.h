@@ -76,6 +76,8 @@ struct DCoax {
     float stack[3], backref[3], gamma;
     F2P f2; F4P t1, t4, t56; F5P phi3, phi4;
     int t1_mode, phi3_enabled, rna_coax; float t1_sa, t1_sb;
+    float t1_cx;  // cos theta1 below which theta1's second piece (mode 0: f4(2 pi - theta1),
+                  // mode 1: SA (theta1 - SB)^2) is nonzero; -2 if never
 };
 struct DDebye { float back[3]; float minus_kappa, prefactor, b_smooth, rc, rhigh; };
 struct DBonded {
@@ -379,8 +381,11 @@ __device__ __forceinline__ float coax_pair(V Rc, const Frame& fi, const Frame& f
     if (fr == 0.f) return 0.f;
     V rsd = rs * (1.f / rsm);
     float c1 = -dot(fi.a1, fj.a1), c4 = dot(fi.a3, fj.a3), c5 = dot(fi.a3, rsd), c6 = -dot(fj.a3, rsd);
-    // theta4, theta5 and theta6 factors screened in cosine space (theta1 has extra modes)
-    if (!(f4_nz(c4, p.t4, false) && f4_nz(c5, p.t56, true) && f4_nz(c6, p.t56, true))) return 0.f;
+    // theta4, theta5, theta6 and theta1 (its f4 or its mode's second piece) factors
+    // screened in cosine space: a zero factor with a zero derivative zeroes the term
+    if (!(f4_nz(c4, p.t4, false) && f4_nz(c5, p.t56, true) && f4_nz(c6, p.t56, true) &&
+          (f4_nz(c1, p.t1, false) || clampc(c1) < p.t1_cx + 4e-6f)))
+        return 0.f;
     float s1, s4, s5, s6;
     float th1 = acos_as(clampc(c1), s1), th4 = acos_as(clampc(c4), s4);
     float th5 = acos_as(clampc(c5), s5), th6 = acos_as(clampc(c6), s6);
