@@ -1596,6 +1596,7 @@ int oxdna_set_state(oxdna_ctx* c, unsigned int n, const double* pos, const doubl
     if (!c) return fail(nullptr, "NULL context");
     return guard(c, [&] {
         if (n == 0) throw Fail("oxdna_set_state: n must be > 0");
+        if (n > kIndexMask) throw Fail("oxdna_set_state: n must be below 2^30 (neighbour rows keep 30-bit indices)");
         if (!pos || !quat || !type || !box) throw Fail("oxdna_set_state: NULL array");
         for (int a = 0; a < 3; ++a)
             if (!(box->L[a] > 0.0)) throw Fail("oxdna_set_state: box lengths must be positive");
@@ -2171,6 +2172,7 @@ int oxdna_pair_forces(int term, float* d_force, float* d_torque, float* d_virial
         if (!d_force || !d_torque || !d_pos || !d_orientation || !d_n_neigh || !d_nlist || !d_head_list)
             throw Fail("oxdna_pair_forces: NULL device array");
         if (term == OXDNA_TERM_DEBYE && !d_charge) throw Fail("oxdna_pair_forces: Debye-Hueckel needs charges");
+        if (N > kIndexMask) throw Fail("oxdna_pair_forces: N must be below 2^30 (neighbour indices keep 30 bits)");
         hipStream_t st = (hipStream_t)stream;
         int dev;
         DriverScratch& s = scratch_for(dev);
@@ -2217,6 +2219,7 @@ int oxdna_bonded_forces(float* d_force, float* d_torque, float* d_virial, size_t
         if (N == 0) return;
         if (!d_force || !d_torque || !d_pos || !d_orientation || !d_blist || !d_bpos || !d_n_bonds)
             throw Fail("oxdna_bonded_forces: NULL device array");
+        if (N > kIndexMask) throw Fail("oxdna_bonded_forces: N must be below 2^30");
         hipStream_t st = (hipStream_t)stream;
         int dev;
         DriverScratch& s = scratch_for(dev);

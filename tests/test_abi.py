@@ -87,6 +87,25 @@ def test_errors_cross_the_boundary_as_status_codes():
     assert lib.oxdna_run(None, 1, 1, 1) != 0
 
 
+def test_drivers_reject_more_particles_than_row_indices_hold():
+    """Neighbour indices keep 30 bits (the row keys' top bits hold the pair class): the
+    stateless drivers reject N >= 2^30 on their arguments, before any device work (the
+    device pointers below are never dereferenced)."""
+    lib = _lib()
+    fake = C.c_void_p(256)
+    box = _abi.Box()
+    box.L[0] = box.L[1] = box.L[2] = 10.0
+    params = (_abi.ExcVolParams * 1)()
+    rcut = (C.c_double * 1)(4.0)
+    rc = lib.oxdna_pair_forces(_abi.TERM_EXCVOL, fake, fake, None, 0, 1 << 30, fake, fake, None, C.byref(box),
+                               fake, fake, fake, 1, params, rcut, None, None)
+    assert rc != 0 and b"2^30" in lib.oxdna_last_error(None)
+    bparams = (_abi.BondedParams * 1)()
+    rc = lib.oxdna_bonded_forces(fake, fake, None, 0, None, 1 << 30, fake, fake, C.byref(box), fake, fake, 1, fake,
+                                 bparams, 1, None)
+    assert rc != 0 and b"2^30" in lib.oxdna_last_error(None)
+
+
 def test_simulation_without_gpu_fails_loudly():
     lib = _lib()
     ctx = C.c_void_p()
