@@ -269,8 +269,11 @@ template<int P, bool WANT_E, bool WANT_V, bool COUNT, bool FUSED = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((FUSED || (WANT_E && WANT_V)) ? 1 : 3)))
 void k_forces(ForceArgs a) {
     constexpr uint32_t TPP = 256 / P;
-    constexpr int NREC = kRecBase + (WANT_V ? 6 : 0) + (WANT_E ? 3 : 0);
-    constexpr int RV = kRecBase, RS = kRecBase + (WANT_V ? 6 : 0);  // virial, split energies
+    // a run step consumes no energy: its records hold force and torque only, and what
+    // only the energies need (the FENE logarithm, ...) is not computed
+    constexpr int NBASE = FUSED ? 6 : kRecBase;
+    constexpr int NREC = NBASE + (WANT_V ? 6 : 0) + (WANT_E ? 3 : 0);
+    constexpr int RV = NBASE, RS = NBASE + (WANT_V ? 6 : 0);  // virial, split energies
     __shared__ double4 s_pos[P];
     __shared__ float4 s_fr[3][P];
     __shared__ size_t s_head[P];
@@ -296,6 +299,18 @@ void k_forces(ForceArgs a) {
 
     double4 pi = make_double4(0, 0, 0, 0);
     if (active) pi = a.pos[i];
+    // the particle's stored row length, near count, row head and frames, loaded before
+    // the rebuild flag is known (a rebuild discards the first three): no load waits for
+    // another before the first barrier
+    uint32_t ld_nn = 0, ld_near = 0;
+    size_t ld_head = 0;
+    float4 ld_f0 = make_float4(0, 0, 0, 0), ld_f1 = ld_f0, ld_f2 = ld_f0;
+    if (active && lane == 0) {
+        ld_nn = a.n_neigh[i];
+        ld_near = a.n_near ? a.n_near[i] : ~0u;
+        ld_head = a.head[i];
+        ld_f0 = a.fr0[i]; ld_f1 = a.fr1[i]; ld_f2 = a.fr2[i];
+    }
     // FUSED: the particle's integrator state, and the part of the merged half-kicks that
     // needs no force (noise, drag), while the first loads are in flight
     double4 e_v, e_p, e_q, e_plast;
@@ -357,8 +372,8 @@ void k_forces(ForceArgs a) {
         nnear = near;
         srow = row;
     } else if (active && lane == 0) {
-        nn = a.n_neigh[i];
-        nnear = a.n_near ? min(a.n_near[i], nn) : nn;  // rows without classes: all near
+        nn = ld_nn;
+        nnear = min(ld_near, nn);  // rows without classes: all near
     }
     float charge_i = 1.f;
     if (lane == 0) {
@@ -367,10 +382,9 @@ void k_forces(ForceArgs a) {
         s_off[1][slot + 1] = active ? nn - nnear : 0u;
         if (active) {
             s_pos[slot] = pi;
-            const float4 f0 = a.fr0[i];
-            s_fr[0][slot] = f0; s_fr[1][slot] = a.fr1[i]; s_fr[2][slot] = a.fr2[i];
-            s_head[slot] = rebuild ? (size_t)slot * 2 * nb.max_neigh + nb.max_neigh : a.head[i];
-            charge_i = f0.w;
+            s_fr[0][slot] = ld_f0; s_fr[1][slot] = ld_f1; s_fr[2][slot] = ld_f2;
+            s_head[slot] = rebuild ? (size_t)slot * 2 * nb.max_neigh + nb.max_neigh : ld_head;
+            charge_i = ld_f0.w;
         }
     }
     __syncthreads();
@@ -436,7 +450,7 @@ void k_forces(ForceArgs a) {
             const float ep = pair_terms<COUNT>(P_, mask, tp, rsq, -dr, fi, fj, f0.w * g0.w, Fp, Tp, et, cnt);
             rec[0] = Fp.x; rec[1] = Fp.y; rec[2] = Fp.z;
             rec[3] = Tp.x; rec[4] = Tp.y; rec[5] = Tp.z;
-            rec[6] = 0.5f * ep;
+            if constexpr (NBASE > 6) rec[6] = 0.5f * ep;
             if (WANT_V) {
                 rec[RV + 0] = 0.5f * dr.x * Fp.x; rec[RV + 1] = 0.5f * dr.x * Fp.y; rec[RV + 2] = 0.5f * dr.x * Fp.z;
                 rec[RV + 3] = 0.5f * dr.y * Fp.y; rec[RV + 4] = 0.5f * dr.y * Fp.z; rec[RV + 5] = 0.5f * dr.z * Fp.z;
@@ -472,7 +486,7 @@ void k_forces(ForceArgs a) {
                 et[S_FENE] += ef;
                 et[S_BEXC] += r.e_bexc;
                 et[S_STACK] += r.e_stack;
-                rec[6] = 0.5f * (ef + r.e_bexc + r.e_stack);
+                if constexpr (NBASE > 6) rec[6] = 0.5f * (ef + r.e_bexc + r.e_stack);
                 if (WANT_V) {
 #pragma unroll
                     for (int q = 0; q < 6; ++q) rec[RV + q] = r.vir[q];
@@ -513,7 +527,9 @@ void k_forces(ForceArgs a) {
     if (WANT_E && a.split && active && lane == 0)
         a.split[i] = make_double4(0.5 * acc[RS + 0], 0.5 * acc[RS + 1], 0.5 * acc[RS + 2], 0.0);
     if (active && lane == 0) {
-        float4 fo = make_float4(acc[0], acc[1], acc[2], acc[6]);
+        float ehalf = 0.f;
+        if constexpr (NBASE > 6) ehalf = acc[6];
+        float4 fo = make_float4(acc[0], acc[1], acc[2], ehalf);
         float4 to = make_float4(acc[3], acc[4], acc[5], 0.f);
         if (FUSED) {
             // Langevin second half of step k with the force just computed, merged with the

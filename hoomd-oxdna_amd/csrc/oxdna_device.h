@@ -674,7 +674,7 @@ __device__ __forceinline__ void bond_eval(const DBonded& P, double Rx, double Ry
         o.e_fene = 1.0e12;  // broken bond: energy sentinel, no force
     } else {
         o.e_fene = -0.5 * P.eps_backbone * log(1.0 - d * d / P.delta2);
-        double pre = -(P.eps_backbone * d / (P.delta2 - d * d)) / rm;
+        double pre = -(P.eps_backbone * d) / ((P.delta2 - d * d) * rm);
         V f = mk((float)(pre * rx), (float)(pre * ry), (float)(pre * rz));
         o.FB += f;
         o.TA -= cross(bA, f);
