@@ -881,10 +881,10 @@ void load_bond_params(DBonded& d, const oxdna_bonded_params& s) {
     d.eps_backbone = s.eps_backbone; d.r0_backbone = s.r0_backbone;
     d.delta_backbone = s.delta_backbone; d.delta2 = s.delta2;
     cvt3(d.back, s.back_site); cvt3(d.base, s.base_site);
-    d.excl_eps = (float)s.excl_eps;
+    d.bx.eps = (float)s.excl_eps;
     for (int k = 0; k < 3; ++k) {
-        d.bx_sigma[k] = (float)s.bx_sigma[k]; d.bx_rstar[k] = (float)s.bx_rstar[k];
-        d.bx_b[k] = (float)s.bx_b[k]; d.bx_rc[k] = (float)s.bx_rc[k];
+        d.bx.sigma[k] = (float)s.bx_sigma[k]; d.bx.rstar[k] = (float)s.bx_rstar[k];
+        d.bx.b[k] = (float)s.bx_b[k]; d.bx.rc[k] = (float)s.bx_rc[k];
     }
     d.stacking_enabled = s.stacking_enabled;
     d.rna_stacking = s.rna_stacking;

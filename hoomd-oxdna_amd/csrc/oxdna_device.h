@@ -80,10 +80,11 @@ struct DCoax {
                   // mode 1: SA (theta1 - SB)^2) is nonzero; -2 if never
 };
 struct DDebye { float back[3]; float minus_kappa, prefactor, b_smooth, rc, rhigh; };
+struct BondExc { float eps, sigma[3], rstar[3], b[3], rc[3]; };
 struct DBonded {
     double eps_backbone, r0_backbone, delta_backbone, delta2;  // FENE stays FP64
     float back[3], base[3];
-    float excl_eps, bx_sigma[3], bx_rstar[3], bx_b[3], bx_rc[3];
+    BondExc bx;  // bonded excluded volume: base-base, base_A-back_B, back_A-base_B
     int stacking_enabled, rna_stacking;
     float stack[3], backref[3], gamma;
     F1P f1; float shift_factor; float st_eps[16];
@@ -111,8 +112,26 @@ struct alignas(16) DParams {  // a multiple of 16 bytes: kernels copy it to LDS 
 };
 
 // ------------------------------------------------------------------ building blocks
+// The force kernel keeps the parameter sets in LDS. A field first read inside a branch
+// is read there and waited for on the spot, one round trip per branch level; eager()
+// reads a whole set (float fields only) up front in one batch with one wait: the empty
+// asm statements take every value as an input, so no read can sink into the branches.
+template<class T>
+__device__ __forceinline__ T eager(const T& p) {
+    static_assert(sizeof(T) % sizeof(float) == 0, "parameter sets hold float fields only");
+    constexpr int n = (int)(sizeof(T) / sizeof(float));
+    T r;
+    const float* src = reinterpret_cast<const float*>(&p);
+    float* dst = reinterpret_cast<float*>(&r);
+#pragma unroll
+    for (int k = 0; k < n; ++k) dst[k] = src[k];
+#pragma unroll
+    for (int k = 0; k < n; ++k) asm volatile("" ::"v"(dst[k]));
+    return r;
+}
 // f1 Morse radial value and derivative (reference src/OxDNANumerics.h:114-155).
-__device__ __forceinline__ void f1_vd(float r, const F1P& s, float eps, float shift, float& v, float& d) {
+__device__ __forceinline__ void f1_vd(float r, const F1P& s_, float eps, float shift, float& v, float& d) {
+    const F1P s = eager(s_);
     v = 0.f; d = 0.f;
     if (r < s.rchigh) {
         if (r > s.rhigh) {
@@ -132,7 +151,8 @@ __device__ __forceinline__ void f1_vd(float r, const F1P& s, float eps, float sh
     }
 }
 // f2 harmonic radial value and derivative (reference src/OxDNANumerics.h:158-192).
-__device__ __forceinline__ void f2_vd(float r, const F2P& s, float& v, float& d) {
+__device__ __forceinline__ void f2_vd(float r, const F2P& s_, float& v, float& d) {
+    const F2P s = eager(s_);
     v = 0.f; d = 0.f;
     if (r < s.rchigh) {
         if (r > s.rhigh) {
@@ -168,14 +188,16 @@ __device__ __forceinline__ void f4_vd(float th, float sth, const F4P& p, float& 
     v = in ? (tail ? v_tail : v_in) : 0.f;
     dsin = in ? (tail ? d_tail : d_in) : 0.f;
 }
-__device__ __forceinline__ float f4_v(float th, const F4P& p) {
+__device__ __forceinline__ float f4_v(float th, const F4P& p_) {
+    const F4P p = eager(p_);
     float t = fabsf(th - p.t0);
     if (!(t < p.tc)) return 0.f;
     if (t > p.ts) { float e = p.tc - t; return p.b * e * e; }
     return 1.f - p.a * t * t;
 }
 // f5 of cos(phi) and its derivative (reference src/OxDNANumerics.h:261-290).
-__device__ __forceinline__ void f5_vd(float c, const F5P& p, float& v, float& d) {
+__device__ __forceinline__ void f5_vd(float c, const F5P& p_, float& v, float& d) {
+    const F5P p = eager(p_);
     v = 0.f; d = 0.f;
     if (c > p.xc) {
         if (c < p.xs) { float e = p.xc - c; v = p.b * e * e; d = 2.f * p.b * (c - p.xc); }
@@ -235,7 +257,8 @@ struct Acc { V F, T; float E; };
 
 // ------------------------------------------------------------------ nonbonded terms
 // Excluded volume (reference src/EvaluatorPairOxDNAExcVol.h:107-149).
-__device__ __forceinline__ float ev_pair(V Rc, const Frame& fi, const Frame& fj, const DExcVol& p, V& F, V& T) {
+__device__ __forceinline__ float ev_pair(V Rc, const Frame& fi, const Frame& fj, const DExcVol& p_, V& F, V& T) {
+    const DExcVol p = eager(p_);
     V bk_i = site(p.back, fi), bs_i = site(p.base, fi);
     V bk_j = site(p.back, fj), bs_j = site(p.base, fj);
     float e = 0.f;
@@ -485,8 +508,9 @@ __device__ __forceinline__ float coax_pair(V Rc, const Frame& fi, const Frame& f
 }
 
 // Debye-Hueckel (reference src/EvaluatorPairOxDNADebye.h:467-526), p = i half.
-__device__ __forceinline__ float dh_pair(V Rc, const Frame& fi, const Frame& fj, float qiqj, const DDebye& p,
+__device__ __forceinline__ float dh_pair(V Rc, const Frame& fi, const Frame& fj, float qiqj, const DDebye& p_,
                                          V& F, V& T) {
+    const DDebye p = eager(p_);
     V bi = site(p.back, fi);
     V rb = Rc + site(p.back, fj) - bi;
     float rm = sqrtf(dot(rb, rb));
@@ -683,13 +707,14 @@ __device__ __forceinline__ void bond_eval(const DBonded& P, double Rx, double Ry
     }
     V Rc = mk((float)Rx, (float)Ry, (float)Rz);
     V eA = site(P.base, fA), eB = site(P.base, fB);
+    const BondExc bx = eager(P.bx);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         V si = (k == 2) ? bA : eA;
         V sj = (k == 1) ? bB : eB;
         V rc = Rc + sj - si;
         V f;
-        o.e_bexc += lj(rc, P.excl_eps, P.bx_sigma[k], P.bx_rstar[k], P.bx_b[k], P.bx_rc[k], f);
+        o.e_bexc += lj(rc, bx.eps, bx.sigma[k], bx.rstar[k], bx.b[k], bx.rc[k], f);
         o.FB += f;
         o.TA -= cross(si, f);
         o.TB += cross(sj, f);
