@@ -65,10 +65,11 @@ __global__ void k_zero_u32(uint32_t* p, size_t n) {
 
 // ------------------------------------------------------------------ neighbour rows
 // Rebuilds particle i's neighbour row inside the force kernel. The TPP lanes of the
-// particle split the (up to) 27 cells around it; each lane counts its matches (FP64
-// minimum-image distance, no contraction, bonded partners excluded), a lane prefix
-// sum places them in the LDS row, and a rank sort writes the row to LDS and HBM
-// (deterministic for any order of the cell members). Row entries are keys: the
+// particle lay out the members of the (up to) 27 cells around it as one list of
+// candidates, take the candidates in turn (FP64 minimum-image distance, no contraction,
+// bonded partners excluded), append the matches to an LDS row, and a rank sort writes
+// the row to LDS and HBM (deterministic for any order of the cell members and of the
+// appends). Row entries are keys: the
 // neighbour index and, in bits 30-31, the terms the pair can reach while the particles
 // move less than the skin: 0 every term, 1 excluded volume and Debye-Hueckel only,
 // 2 Debye-Hueckel only. Sorting by key groups the classes, so a pass of the lanes over
@@ -97,31 +98,76 @@ __device__ __forceinline__ int ref_pair(const NbrArgs& nb, const double4& pi, ui
     return r2 < nb.rbb2 ? 1 : (r2 < nb.rnear2 ? 2 : 3);
 }
 
+// The (up to) 27 cells around particle i and where each cell's members start in the
+// particle's list of candidates: lane `lane` of the TPP takes CPL consecutive cells, reads
+// their occupancies, and a lane prefix sum places them. c_id[q] receives cell q's index,
+// c_st[q] its first candidate (c_st[q] = the candidate count for q >= the cell count, up
+// to q = 27).
 template<int TPP>
-__device__ __forceinline__ uint32_t scan_cells(const NbrArgs& nb, const double4& pi, uint32_t i, int b0, int b1,
-                                               uint32_t lane, uint32_t* row, uint32_t base, bool write) {
+__device__ __forceinline__ void cell_starts(const NbrArgs& nb, const double4& pi, uint32_t lane, uint32_t* c_id,
+                                            uint32_t* c_st) {
+    constexpr int CPL = (27 + TPP - 1) / TPP;
     const int nx = nb.nc[0] >= 3 ? 3 : nb.nc[0], ny = nb.nc[1] >= 3 ? 3 : nb.nc[1], nz = nb.nc[2] >= 3 ? 3 : nb.nc[2];
     const int ox = nb.nc[0] >= 3 ? -1 : 0, oy = nb.nc[1] >= 3 ? -1 : 0, oz = nb.nc[2] >= 3 ? -1 : 0;
+    const int ncell = nx * ny * nz;
     const uint32_t c = cell_index(pi.x, pi.y, pi.z, nb.Linv, nb.nc);
     const int cx = c % nb.nc[0], cy = (c / nb.nc[0]) % nb.nc[1], cz = c / (nb.nc[0] * nb.nc[1]);
-    uint32_t n = 0;
-    for (int q = lane; q < nx * ny * nz; q += TPP) {
-        int x = (cx + ox + q % nx + nb.nc[0]) % nb.nc[0];
-        int y = (cy + oy + (q / nx) % ny + nb.nc[1]) % nb.nc[1];
-        int z = (cz + oz + q / (nx * ny) + nb.nc[2]) % nb.nc[2];
-        uint32_t cc = ((uint32_t)z * nb.nc[1] + y) * nb.nc[0] + x;
-        uint32_t m = min(nb.count[cc], nb.cap);
-        const uint32_t* mem = nb.members + (size_t)cc * nb.cap;
-        for (uint32_t k = 0; k < m; ++k) {
-            const uint32_t j = mem[k];
-            const int cls = ref_pair<TPP>(nb, pi, i, j, b0, b1);
-            if (cls != 0) {
-                if (write && base + n < nb.max_neigh) row[base + n] = j | ((uint32_t)(cls - 1) << kClassShift);
-                ++n;
-            }
+    uint32_t cnt[CPL], sum = 0;
+#pragma unroll
+    for (int u = 0; u < CPL; ++u) {
+        const int q = (int)lane * CPL + u;
+        cnt[u] = 0;
+        if (q < ncell) {
+            int x = (cx + ox + q % nx + nb.nc[0]) % nb.nc[0];
+            int y = (cy + oy + (q / nx) % ny + nb.nc[1]) % nb.nc[1];
+            int z = (cz + oz + q / (nx * ny) + nb.nc[2]) % nb.nc[2];
+            const uint32_t cc = ((uint32_t)z * nb.nc[1] + y) * nb.nc[0] + x;
+            c_id[q] = cc;
+            cnt[u] = min(nb.count[cc], nb.cap);
+        }
+        sum += cnt[u];
+    }
+    uint32_t v = sum;
+#pragma unroll
+    for (uint32_t d = 1; d < TPP; d <<= 1) {
+        uint32_t y = __shfl_up(v, d, TPP);
+        if (lane >= d) v += y;
+    }
+    uint32_t st = v - sum;
+#pragma unroll
+    for (int u = 0; u < CPL; ++u) {
+        const int q = (int)lane * CPL + u;
+        if (q <= 27) c_st[q] = st;
+        st += cnt[u];
+    }
+}
+
+// The neighbours of particle i among its cells' members: the TPP lanes take the
+// candidates in turn (a dense cell is shared by all lanes instead of walked by one) and
+// append matches to urow through an LDS counter. The order of urow depends on the
+// counter; the rank sort that follows makes the row independent of it.
+template<int TPP>
+__device__ __forceinline__ void row_candidates(const NbrArgs& nb, const double4& pi, uint32_t i, int b0, int b1,
+                                               uint32_t lane, const uint32_t* c_id, const uint32_t* c_st,
+                                               uint32_t* urow, uint32_t* counter) {
+    const uint32_t T = c_st[27];
+    for (uint32_t t = lane; t < T; t += TPP) {
+        // cell of candidate t: the largest q with c_st[q] <= t (empty cells are skipped)
+        uint32_t lo = 0, n = 27;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            if (n <= 1) break;
+            const uint32_t h = n / 2;
+            if (c_st[lo + h] <= t) lo += h;
+            n -= h;
+        }
+        const uint32_t j = nb.members[(size_t)c_id[lo] * nb.cap + (t - c_st[lo])];
+        const int cls = ref_pair<TPP>(nb, pi, i, j, b0, b1);
+        if (cls != 0) {
+            const uint32_t p = atomicAdd(counter, 1u);
+            if (p < nb.max_neigh) urow[p] = j | ((uint32_t)(cls - 1) << kClassShift);
         }
     }
-    return n;
 }
 
 // ------------------------------------------------------------------ integrator
@@ -278,6 +324,7 @@ void k_forces(ForceArgs a) {
     __shared__ float4 s_fr[3][P];
     __shared__ size_t s_head[P];
     __shared__ uint32_t s_nnear[P];
+    __shared__ uint32_t s_cnt[P];  // rebuild launches: each particle's row length so far
     __shared__ uint32_t s_off[2][P + 1];  // exclusive prefix sums of near / far counts
     __shared__ float s_rec[NREC][256];
     // The term parameters, staged in LDS: the inlined term evaluators reach them through
@@ -328,21 +375,20 @@ void k_forces(ForceArgs a) {
         // neighbour row of i (uniform branch: every thread of the block is here)
         uint32_t* urow = lds_rows + (size_t)slot * 2 * nb.max_neigh;
         uint32_t* row = urow + nb.max_neigh;
+        // the cell ids and candidate starts sit in `row` (at least 64 entries) until the
+        // rank sort fills it
+        uint32_t* c_id = row;
+        uint32_t* c_st = row + 32;
         int b0 = -1, b1 = -1;
-        uint32_t cnt = 0;
+        if (lane == 0) s_cnt[slot] = 0u;
         if (active) {
             b0 = a.bpart[2 * i]; b1 = a.bpart[2 * i + 1];
-            cnt = scan_cells<TPP>(nb, pi, i, b0, b1, lane, urow, 0, false);
+            cell_starts<TPP>(nb, pi, lane, c_id, c_st);
         }
-        uint32_t v = cnt;
-#pragma unroll
-        for (uint32_t d = 1; d < TPP; d <<= 1) {
-            uint32_t y = __shfl_up(v, d, TPP);
-            if (lane >= d) v += y;
-        }
-        const uint32_t total = __shfl(v, TPP - 1, TPP);
-        if (active) scan_cells<TPP>(nb, pi, i, b0, b1, lane, urow, v - cnt, true);
         __syncthreads();
+        if (active) row_candidates<TPP>(nb, pi, i, b0, b1, lane, c_id, c_st, urow, &s_cnt[slot]);
+        __syncthreads();
+        const uint32_t total = active ? s_cnt[slot] : 0u;
         const uint32_t m = min(total, nb.max_neigh);
         uint32_t near = 0;
         if (active) {
@@ -1100,6 +1146,9 @@ ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild, u
     nb.rnear2 = ((mask >> T_DEBYE) & 1u) ? (rnear + c->buffer) * (rnear + c->buffer) : nb.rlist2;
     nb.rbb2 = ((mask >> T_EXCVOL) & 1u) ? (rbb + c->buffer) * (rbb + c->buffer) : nb.rnear2;
     nb.max_neigh = c->max_neigh;
+    // an in-kernel rebuild keeps the cell ids and candidate starts in the sorted row's
+    // LDS space until the sort: 60 entries (rows start at 64 and only grow)
+    if (rebuild != 0 && c->max_neigh < 64) throw Fail("neighbour rows below 64 entries");
     nb.pos_last = c->pos_last.p;
     nb.flags = (rebuild != 0 && (mask & 0x3fu)) ? c->flags.p : nullptr;
     const bool pairs = (mask & 0x3fu) != 0;
