@@ -174,12 +174,12 @@ __device__ __forceinline__ void f2_vd(float r, const F2P& s_, float& v, float& d
 // (reference src/OxDNANumerics.h:201-244, inner-branch 1e-8 guard included).
 // Branch-free: both pieces are computed and selected (a wavefront's lanes would run
 // both branches anyway).
-__device__ __forceinline__ void f4_vd(float th, float sth, const F4P& p, float& v, float& dsin) {
+// inv = 1 / sth, passed in where several factors share one angle's sine.
+__device__ __forceinline__ void f4_vdi(float th, float sth, float inv, const F4P& p, float& v, float& dsin) {
     const float t0 = th - p.t0;
     const float t = fabsf(t0);
     const float m = t0 < 0.f ? -1.f : 1.f;
     const float e = p.tc - t;
-    const float inv = 1.f / sth;
     const bool tail = t > p.ts;
     const float v_tail = p.b * e * e, d_tail = -m * 2.f * p.b * e * inv;
     const float v_in = 1.f - p.a * t * t;
@@ -187,6 +187,9 @@ __device__ __forceinline__ void f4_vd(float th, float sth, const F4P& p, float& 
     const bool in = t < p.tc;
     v = in ? (tail ? v_tail : v_in) : 0.f;
     dsin = in ? (tail ? d_tail : d_in) : 0.f;
+}
+__device__ __forceinline__ void f4_vd(float th, float sth, const F4P& p, float& v, float& dsin) {
+    f4_vdi(th, sth, 1.f / sth, p, v, dsin);
 }
 __device__ __forceinline__ float f4_v(float th, const F4P& p_) {
     const F4P p = eager(p_);
@@ -217,23 +220,6 @@ __device__ __forceinline__ bool f4_nz(float c, const F4P& p, bool sym) {
     return nz;
 }
 
-// Angular factor with optional reflection (reference src/OxDNABaseBase.h:52-67):
-// value f4(th) [+ f4(pi - th)], coefficient sign * (D(th) [- D(pi - th)]).
-struct Ang { float v, d; };
-__device__ __forceinline__ Ang angular(float c, const F4P& p, bool sym, float sign) {
-    float s;
-    float th = acos_as(clampc(c), s);
-    Ang a;
-    f4_vd(th, s, p, a.v, a.d);
-    if (sym) {
-        float v2, d2;
-        f4_vd(3.14159265358979f - th, s, p, v2, d2);
-        a.v += v2;
-        a.d -= d2;
-    }
-    a.d *= sign;
-    return a;
-}
 
 // Smoothed LJ core (reference src/OxDNANumerics.h:76-107); f = force on the q site.
 __device__ __forceinline__ float lj(V r, float eps, float sigma, float rstar, float b, float rc, V& f) {
@@ -275,13 +261,14 @@ __device__ __forceinline__ float ev_pair(V Rc, const Frame& fi, const Frame& fj,
 }
 
 // Shared BASE-BASE geometry (reference src/OxDNABaseBase.h:84-121).
-struct BB { V rdir, base_i; float rmod, c1, c2, c3, c4, c7, c8; };
+struct BB { V rdir, base_i; float rmod, irmod, c1, c2, c3, c4, c7, c8; };  // irmod = 1 / rmod
 __device__ __forceinline__ bool bb_geom(V Rc, const Frame& fi, const Frame& fj, const float base[3], BB& g) {
     g.base_i = site(base, fi);
     V r = Rc + site(base, fj) - g.base_i;
     g.rmod = sqrtf(dot(r, r));
     if (!(g.rmod > 0.f)) return false;
-    g.rdir = r * (1.f / g.rmod);
+    g.irmod = 1.f / g.rmod;
+    g.rdir = r * g.irmod;
     g.c1 = -dot(fi.a1, fj.a1);
     g.c2 = -dot(fj.a1, g.rdir);
     g.c3 = dot(fi.a1, g.rdir);
@@ -294,24 +281,35 @@ __device__ __forceinline__ bool bb_geom(V Rc, const Frame& fi, const Frame& fj, 
 // The angles of the six BASE-BASE cosines (c1, c2, c3, c4, c7, c8) and their sines: one
 // arccos each, shared by the two terms of the fused hydrogen-bonding + cross-stacking
 // evaluator, which read the same geometry.
-struct BBAng { float th[6], s[6]; };
+struct BBAng { float th[6], s[6], inv[6]; };  // angles, sines, 1 / sines
 __device__ __forceinline__ void bb_angles(const BB& g, BBAng& a) {
     const float c[6] = {g.c1, g.c2, g.c3, g.c4, g.c7, g.c8};
 #pragma unroll
-    for (int k = 0; k < 6; ++k) a.th[k] = acos_as(clampc(c[k]), a.s[k]);
+    for (int k = 0; k < 6; ++k) {
+        a.th[k] = acos_as(clampc(c[k]), a.s[k]);
+        a.inv[k] = 1.f / a.s[k];
+    }
 }
-// Angular factor from a precomputed angle (angular() without the arccos).
-__device__ __forceinline__ Ang angular_th(float th, float s, const F4P& p, bool sym, float sign) {
+// Angular factor with optional reflection (reference src/OxDNABaseBase.h:52-67):
+// value f4(th) [+ f4(pi - th)], coefficient sign * (D(th) [- D(pi - th)]).
+struct Ang { float v, d; };
+// Angular factor from a precomputed angle, its sine and the sine's inverse.
+__device__ __forceinline__ Ang angular_th(float th, float s, float inv, const F4P& p, bool sym, float sign) {
     Ang a;
-    f4_vd(th, s, p, a.v, a.d);
+    f4_vdi(th, s, inv, p, a.v, a.d);
     if (sym) {
         float v2, d2;
-        f4_vd(3.14159265358979f - th, s, p, v2, d2);
+        f4_vdi(3.14159265358979f - th, s, inv, p, v2, d2);
         a.v += v2;
         a.d -= d2;
     }
     a.d *= sign;
     return a;
+}
+__device__ __forceinline__ Ang angular(float c, const F4P& p, bool sym, float sign) {
+    float s;
+    float th = acos_as(clampc(c), s);
+    return angular_th(th, s, 1.f / s, p, sym, sign);
 }
 
 // Whether a BASE-BASE term can be nonzero: nonzero radial (the reference's early exit)
@@ -334,19 +332,19 @@ __device__ __forceinline__ void bbw_add(BBW& w, const BBW& x) {
 // src/OxDNABaseBase.h:128-219, the p = i half); zero weights when a factor vanishes.
 __device__ __forceinline__ BBW bb_weights(const BB& g, const BBAng& an, float rv, float rd, const F4P& t1,
                                           const F4P& t23, const F4P& t4, const F4P& t78, bool sym, bool use_t4) {
-    Ang A1 = angular_th(an.th[0], an.s[0], t1, false, -1.f);
-    Ang A2 = angular_th(an.th[1], an.s[1], t23, false, -1.f);
-    Ang A3 = angular_th(an.th[2], an.s[2], t23, false, 1.f);
-    Ang A4 = use_t4 ? angular_th(an.th[3], an.s[3], t4, sym, 1.f) : Ang{1.f, 0.f};
-    Ang A7 = angular_th(an.th[4], an.s[4], t78, sym, -1.f);
-    Ang A8 = angular_th(an.th[5], an.s[5], t78, sym, 1.f);
+    Ang A1 = angular_th(an.th[0], an.s[0], an.inv[0], t1, false, -1.f);
+    Ang A2 = angular_th(an.th[1], an.s[1], an.inv[1], t23, false, -1.f);
+    Ang A3 = angular_th(an.th[2], an.s[2], an.inv[2], t23, false, 1.f);
+    Ang A4 = use_t4 ? angular_th(an.th[3], an.s[3], an.inv[3], t4, sym, 1.f) : Ang{1.f, 0.f};
+    Ang A7 = angular_th(an.th[4], an.s[4], an.inv[4], t78, sym, -1.f);
+    Ang A8 = angular_th(an.th[5], an.s[5], an.inv[5], t78, sym, 1.f);
     float p12 = A1.v * A2.v, p34 = A3.v * A4.v, p78 = A7.v * A8.v;
     float prod = p12 * p34 * p78;
     BBW w;
     w.e = rv * prod;
     if (w.e == 0.f) { bbw_zero(w); return w; }
     // products with one factor replaced by its coefficient
-    const float inv = 1.f / g.rmod;
+    const float inv = g.irmod;
     w.wr = -rd * prod;
     w.w1 = rv * A1.d * A2.v * p34 * p78;
     w.w2 = rv * A1.v * A2.d * p34 * p78 * inv;
@@ -413,25 +411,26 @@ __device__ __forceinline__ float coax_pair(V Rc, const Frame& fi, const Frame& f
     float th1 = acos_as(clampc(c1), s1), th4 = acos_as(clampc(c4), s4);
     float th5 = acos_as(clampc(c5), s5), th6 = acos_as(clampc(c6), s6);
     float g1, g1D, v, d;
-    f4_vd(th1, s1, p.t1, g1, g1D);
+    const float inv1 = 1.f / s1, inv5 = 1.f / s5, inv6 = 1.f / s6;  // each shared by two factors
+    f4_vdi(th1, s1, inv1, p.t1, g1, g1D);
     g1D = -g1D;
     if (p.t1_mode == 0) {
-        f4_vd(2.f * PI - th1, -s1, p.t1, v, d);  // sin(2 pi - t) = -sin(t)
+        f4_vdi(2.f * PI - th1, -s1, -inv1, p.t1, v, d);  // sin(2 pi - t) = -sin(t)
         g1 += v; g1D -= d;
     } else if (th1 > p.t1_sb) {
         float t = th1 - p.t1_sb;
         g1 += p.t1_sa * t * t;
-        g1D -= (s1 * s1 > 1e-8f) ? 2.f * p.t1_sa * t / s1 : 2.f * p.t1_sa;
+        g1D -= (s1 * s1 > 1e-8f) ? 2.f * p.t1_sa * t * inv1 : 2.f * p.t1_sa;
     }
     float g4, g4D;
     f4_vd(th4, s4, p.t4, g4, g4D);
     float g5, g5D, g6, g6D;
-    f4_vd(th5, s5, p.t56, g5, g5D);
-    f4_vd(PI - th5, s5, p.t56, v, d);
+    f4_vdi(th5, s5, inv5, p.t56, g5, g5D);
+    f4_vdi(PI - th5, s5, inv5, p.t56, v, d);
     g5 += v; g5D -= d;
-    f4_vd(th6, s6, p.t56, g6, g6D);
+    f4_vdi(th6, s6, inv6, p.t56, g6, g6D);
     g6D = -g6D;
-    f4_vd(PI - th6, s6, p.t56, v, d);
+    f4_vdi(PI - th6, s6, inv6, p.t56, v, d);
     g6 += v; g6D += d;
     V br_i = site(p.backref, fi), br_j = site(p.backref, fj);
     V rb = Rc + br_j - br_i;
