@@ -71,17 +71,21 @@ __global__ void k_zero_u32(uint32_t* p, size_t n) {
 // the row to LDS and HBM (deterministic for any order of the cell members and of the
 // appends). Row entries are keys: the
 // neighbour index and, in bits 30-31, the terms the pair can reach while the particles
-// move less than the skin: 0 every term, 1 excluded volume and Debye-Hueckel only,
-// 2 Debye-Hueckel only. Sorting by key groups the classes, so a pass of the lanes over
-// the row mostly runs one class's path instead of the union of all of them. Lanes of one
+// move less than the skin: 0 and 1 every term ("near"; 0: the base-base radial reaches
+// the pair at the rebuild already), 2 excluded volume and Debye-Hueckel only, 3
+// Debye-Hueckel only. Sorting by key groups the classes, so a wavefront mostly runs one
+// class's path instead of the union of all of them; the classes only order the work,
+// every entry is evaluated for every selected term. Lanes of one
 // particle are adjacent lanes of one wavefront; every thread of the block reaches the
 // barriers.
 constexpr uint32_t kClassShift = 30;
 constexpr uint32_t kIndexMask = (1u << kClassShift) - 1u;
 
-// 0: not a neighbour; else 1 + the pair's class (see above).
+// 0: not a neighbour; else 1 + the pair's class (see above). bsi: i's base site (lab
+// frame, relative to i).
 template<int TPP>
-__device__ __forceinline__ int ref_pair(const NbrArgs& nb, const double4& pi, uint32_t i, uint32_t j, int b0, int b1) {
+__device__ __forceinline__ int ref_pair(const NbrArgs& nb, const double4& pi, const V& bsi, uint32_t i, uint32_t j,
+                                        int b0, int b1) {
     if (j == i || (int)j == b0 || (int)j == b1) return 0;
     const double4 pj = nb.pos[j];
     double dx = min_image(pi.x - pj.x, nb.L[0], nb.Linv[0]);
@@ -95,7 +99,19 @@ __device__ __forceinline__ int ref_pair(const NbrArgs& nb, const double4& pi, ui
         r2 = r2 + dz * dz;
     }
     if (!(r2 < nb.rlist2)) return 0;
-    return r2 < nb.rbb2 ? 1 : (r2 < nb.rnear2 ? 2 : 3);
+    if (r2 < nb.rbb2) {
+        if (nb.bb_act2 > 0.f) {
+            // base site of j minus that of i (bsi), r_j - r_i = -(dx, dy, dz)
+            const float4 g0 = nb.fr0[j], g1 = nb.fr1[j], g2 = nb.fr2[j];
+            const float b0x = nb.bb_base[0], b1x = nb.bb_base[1], b2x = nb.bb_base[2];
+            const float sx = (float)(-dx) + b0x * g0.x + b1x * g1.x + b2x * g2.x - bsi.x;
+            const float sy = (float)(-dy) + b0x * g0.y + b1x * g1.y + b2x * g2.y - bsi.y;
+            const float sz = (float)(-dz) + b0x * g0.z + b1x * g1.z + b2x * g2.z - bsi.z;
+            if (sx * sx + sy * sy + sz * sz < nb.bb_act2) return 1;
+        }
+        return 2;
+    }
+    return r2 < nb.rnear2 ? 3 : 4;
 }
 
 // The (up to) 27 cells around particle i and where each cell's members start in the
@@ -147,7 +163,8 @@ __device__ __forceinline__ void cell_starts(const NbrArgs& nb, const double4& pi
 // append matches to urow through an LDS counter. The order of urow depends on the
 // counter; the rank sort that follows makes the row independent of it.
 template<int TPP>
-__device__ __forceinline__ void row_candidates(const NbrArgs& nb, const double4& pi, uint32_t i, int b0, int b1,
+__device__ __forceinline__ void row_candidates(const NbrArgs& nb, const double4& pi, const V& bsi, uint32_t i, int b0,
+                                               int b1,
                                                uint32_t lane, const uint32_t* c_id, const uint32_t* c_st,
                                                uint32_t* urow, uint32_t* counter) {
     const uint32_t T = c_st[27];
@@ -162,7 +179,7 @@ __device__ __forceinline__ void row_candidates(const NbrArgs& nb, const double4&
             n -= h;
         }
         const uint32_t j = nb.members[(size_t)c_id[lo] * nb.cap + (t - c_st[lo])];
-        const int cls = ref_pair<TPP>(nb, pi, i, j, b0, b1);
+        const int cls = ref_pair<TPP>(nb, pi, bsi, i, j, b0, b1);
         if (cls != 0) {
             const uint32_t p = atomicAdd(counter, 1u);
             if (p < nb.max_neigh) urow[p] = j | ((uint32_t)(cls - 1) << kClassShift);
@@ -325,7 +342,7 @@ void k_forces(ForceArgs a) {
     __shared__ size_t s_head[P];
     __shared__ uint32_t s_nnear[P];
     __shared__ uint32_t s_cnt[P];  // rebuild launches: each particle's row length so far
-    __shared__ uint32_t s_off[2][P + 1];  // exclusive prefix sums of near / far counts
+    __shared__ uint32_t s_off[3][P + 1];  // exclusive prefix sums of class-0 near / class-1 near / far counts
     __shared__ float s_rec[NREC][256];
     // The term parameters, staged in LDS: the inlined term evaluators reach them through
     // addresses the compiler may select per lane (one code path serving the hydrogen-
@@ -354,7 +371,7 @@ void k_forces(ForceArgs a) {
     float4 ld_f0 = make_float4(0, 0, 0, 0), ld_f1 = ld_f0, ld_f2 = ld_f0;
     if (active && lane == 0) {
         ld_nn = a.n_neigh[i];
-        ld_near = a.n_near ? a.n_near[i] : ~0u;
+        ld_near = a.n_near ? a.n_near[i] : 0xffffu;  // rows without classes: all near, class 1
         ld_head = a.head[i];
         ld_f0 = a.fr0[i]; ld_f1 = a.fr1[i]; ld_f2 = a.fr2[i];
     }
@@ -369,7 +386,7 @@ void k_forces(ForceArgs a) {
     }
 
     // ---- (1) rows
-    uint32_t nn = 0, nnear = 0;
+    uint32_t nn = 0, nnear = 0, nA = 0;  // row length, near entries, of them class 0
     const uint32_t* srow = nullptr;
     if (rebuild) {
         // neighbour row of i (uniform branch: every thread of the block is here)
@@ -380,17 +397,23 @@ void k_forces(ForceArgs a) {
         uint32_t* c_id = row;
         uint32_t* c_st = row + 32;
         int b0 = -1, b1 = -1;
+        V bsi = zero3();
         if (lane == 0) s_cnt[slot] = 0u;
         if (active) {
             b0 = a.bpart[2 * i]; b1 = a.bpart[2 * i + 1];
+            if (nb.bb_act2 > 0.f) {
+                const float4 f0 = a.fr0[i], f1 = a.fr1[i], f2 = a.fr2[i];
+                Frame fi{mk(f0.x, f0.y, f0.z), mk(f1.x, f1.y, f1.z), mk(f2.x, f2.y, f2.z)};
+                bsi = site(nb.bb_base, fi);
+            }
             cell_starts<TPP>(nb, pi, lane, c_id, c_st);
         }
         __syncthreads();
-        if (active) row_candidates<TPP>(nb, pi, i, b0, b1, lane, c_id, c_st, urow, &s_cnt[slot]);
+        if (active) row_candidates<TPP>(nb, pi, bsi, i, b0, b1, lane, c_id, c_st, urow, &s_cnt[slot]);
         __syncthreads();
         const uint32_t total = active ? s_cnt[slot] : 0u;
         const uint32_t m = min(total, nb.max_neigh);
-        uint32_t near = 0;
+        uint32_t near = 0, nearA = 0;
         if (active) {
             for (uint32_t e = lane; e < m; e += TPP) {
                 uint32_t x = urow[e], r = 0;
@@ -398,14 +421,18 @@ void k_forces(ForceArgs a) {
                 for (uint32_t q = 0; q < m; ++q) r += (urow[q] < x || (urow[q] == x && q < e)) ? 1u : 0u;
                 row[r] = x;
                 a.nlist[a.head[i] + r] = x;
-                near += (x >> kClassShift) == 0u ? 1u : 0u;
+                near += (x >> kClassShift) < 2u ? 1u : 0u;
+                nearA += (x >> kClassShift) == 0u ? 1u : 0u;
             }
         }
 #pragma unroll
-        for (uint32_t d = TPP / 2; d > 0; d >>= 1) near += __shfl_xor(near, d, TPP);
+        for (uint32_t d = TPP / 2; d > 0; d >>= 1) {
+            near += __shfl_xor(near, d, TPP);
+            nearA += __shfl_xor(nearA, d, TPP);
+        }
         if (active && lane == 0) {
             a.n_neigh[i] = m;
-            if (a.n_near) a.n_near[i] = near;
+            if (a.n_near) a.n_near[i] = near | (nearA << 16);
             nb.pos_last[i] = pi;
             if (total > nb.max_neigh) atomicOr(&nb.flags->overflow, 2u);
             // the largest count sizes the rows: wanted from a forced rebuild (which
@@ -416,16 +443,19 @@ void k_forces(ForceArgs a) {
         }
         nn = m;
         nnear = near;
+        nA = nearA;
         srow = row;
     } else if (active && lane == 0) {
         nn = ld_nn;
-        nnear = min(ld_near, nn);  // rows without classes: all near
+        nnear = min(ld_near & 0xffffu, nn);
+        nA = min(ld_near >> 16, nnear);
     }
     float charge_i = 1.f;
     if (lane == 0) {
         s_nnear[slot] = active ? nnear : 0u;
-        s_off[0][slot + 1] = active ? nnear : 0u;
-        s_off[1][slot + 1] = active ? nn - nnear : 0u;
+        s_off[0][slot + 1] = active ? nA : 0u;
+        s_off[1][slot + 1] = active ? nnear - nA : 0u;
+        s_off[2][slot + 1] = active ? nn - nnear : 0u;
         if (active) {
             s_pos[slot] = pi;
             s_fr[0][slot] = ld_f0; s_fr[1][slot] = ld_f1; s_fr[2][slot] = ld_f2;
@@ -436,7 +466,7 @@ void k_forces(ForceArgs a) {
     __syncthreads();
     if (t < 64) {  // exclusive prefix sums over the P particles (P <= 64: one wavefront)
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
+        for (int c = 0; c < 3; ++c) {
             uint32_t v = t < P ? s_off[c][t + 1] : 0u;
 #pragma unroll
             for (uint32_t d = 1; d < 64; d <<= 1) {
@@ -448,7 +478,9 @@ void k_forces(ForceArgs a) {
         }
     }
     __syncthreads();
-    const uint32_t NN = s_off[0][P], NF = s_off[1][P];
+    // near entries: every particle's class-0 entries, then every particle's class-1
+    // entries (the base-base path runs on the wavefronts of the first part)
+    const uint32_t NA = s_off[0][P], NN = NA + s_off[1][P], NF = s_off[2][P];
     const uint32_t NB = (mask & (1u << T_BONDED)) ? ((2u * P + 63u) & ~63u) : 0u;
     const uint32_t B0 = (NN + 63u) & ~63u, F0 = B0 + NB, NT = F0 + NF;
 
@@ -470,13 +502,18 @@ void k_forces(ForceArgs a) {
         for (int k = 0; k < NREC; ++k) rec[k] = 0.f;
         bool pair = false;
         uint32_t s = 0, k = 0;
-        if (e < NN) {
+        if (e < NA) {
             s = owner_of<P>(s_off[0], e);
             k = e - s_off[0][s];
             pair = true;
+        } else if (e < NN) {
+            const uint32_t x = e - NA;
+            s = owner_of<P>(s_off[1], x);
+            k = (s_off[0][s + 1] - s_off[0][s]) + (x - s_off[1][s]);
+            pair = true;
         } else if (e >= F0 && e < NT) {
-            s = owner_of<P>(s_off[1], e - F0);
-            k = s_nnear[s] + (e - F0 - s_off[1][s]);
+            s = owner_of<P>(s_off[2], e - F0);
+            k = s_nnear[s] + (e - F0 - s_off[2][s]);
             pair = true;
         }
         if (pair) {
@@ -546,10 +583,11 @@ void k_forces(ForceArgs a) {
         // this particle's records of the round: near entries, bond slots, far entries
         if (active) {
             const uint32_t r1 = r0 + 256;
-            uint32_t lo[3] = {s_off[0][slot], B0 + 2 * slot, F0 + s_off[1][slot]};
-            uint32_t hi[3] = {s_off[0][slot + 1], NB ? B0 + 2 * slot + 2 : B0 + 2 * slot, F0 + s_off[1][slot + 1]};
+            uint32_t lo[4] = {s_off[0][slot], NA + s_off[1][slot], B0 + 2 * slot, F0 + s_off[2][slot]};
+            uint32_t hi[4] = {s_off[0][slot + 1], NA + s_off[1][slot + 1], NB ? B0 + 2 * slot + 2 : B0 + 2 * slot,
+                              F0 + s_off[2][slot + 1]};
 #pragma unroll
-            for (int g = 0; g < 3; ++g) {
+            for (int g = 0; g < 4; ++g) {
                 const uint32_t x0 = max(lo[g], r0), x1 = min(hi[g], r1);
                 for (uint32_t x = x0 + lane; x < x1; x += TPP) {
 #pragma unroll
@@ -1145,6 +1183,19 @@ ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild, u
     }
     nb.rnear2 = ((mask >> T_DEBYE) & 1u) ? (rnear + c->buffer) * (rnear + c->buffer) : nb.rlist2;
     nb.rbb2 = ((mask >> T_EXCVOL) & 1u) ? (rbb + c->buffer) * (rbb + c->buffer) : nb.rnear2;
+    // near pairs within the base-base radial reach at the rebuild go first (class 0)
+    nb.fr0 = a.fr0; nb.fr1 = a.fr1; nb.fr2 = a.fr2;
+    nb.bb_act2 = 0.f;
+    const DBaseBase* bb = ((mask >> T_HBONDCROSS) & 1u) ? &c->P.hbx
+                          : ((mask >> T_HBOND) & 1u) ? &c->P.hb : ((mask >> T_CROSS) & 1u) ? &c->P.cs : nullptr;
+    if (bb) {
+        float reach = 0.f;
+        if ((mask >> T_HBONDCROSS) & 1u) reach = std::max(bb->f1.rchigh, bb->f2.rchigh);
+        if ((mask >> T_HBOND) & 1u) reach = std::max(reach, c->P.hb.f1.rchigh);
+        if ((mask >> T_CROSS) & 1u) reach = std::max(reach, c->P.cs.f2.rchigh);
+        for (int d = 0; d < 3; ++d) nb.bb_base[d] = bb->base[d];
+        nb.bb_act2 = reach * reach;
+    }
     nb.max_neigh = c->max_neigh;
     // an in-kernel rebuild keeps the cell ids and candidate starts in the sorted row's
     // LDS space until the sort: 60 entries (rows start at 64 and only grow)

@@ -196,6 +196,14 @@ struct NbrArgs {
     const uint32_t* members;
     double rlist2;
     double rbb2;           // closer than this (squared): every term can reach the pair
+    // frames of the particles and the base site of the base-base terms: a near pair whose
+    // base sites are closer than sqrt(bb_act2) at the rebuild (its base-base radial
+    // reaches it now) is class 0, other near pairs class 1 (bb_act2 <= 0: all class 1)
+    const float4* fr0;
+    const float4* fr1;
+    const float4* fr2;
+    float bb_base[3];
+    float bb_act2;
     double rnear2;         // closer than this: excluded volume + Debye-Hueckel; else Debye-Hueckel only
     uint32_t max_neigh;
     double4* pos_last;
@@ -241,7 +249,8 @@ struct ForceArgs {
     const float4* fr1;     // a2, type (int bits)
     const float4* fr2;     // a3, unused
     uint32_t* n_neigh;
-    uint32_t* n_near;      // leading class-0 entries of each row (NULL: rows without classes, all near)
+    uint32_t* n_near;      // per row: near entries (classes 0-1, low 16 bits), of them class 0 (high 16 bits);
+                           // NULL: rows without classes, all near
     uint32_t* nlist;
     const size_t* head;
     const int* bpart;      // 2 per particle: partner or -1
