@@ -297,10 +297,11 @@ __global__ void k_step2(StepArgs a) {
 // ------------------------------------------------------------------ fused forces
 // A workgroup of 256 threads owns P consecutive particles (TPP = 256 / P lanes each in
 // the per-particle phases). (1) Each particle's neighbour row, rebuilt in-kernel when the
-// skin test asked for it; its leading class-0 entries are "near" (every term), the rest
-// "far" (excluded volume and Debye-Hueckel at most). (2) The workgroup's near entries,
-// its bond slots and its far entries, laid end to end with each part starting on a
-// wavefront boundary, are evaluated one per thread, 256 per round, so the base-base,
+// skin test asked for it; its leading class-0 and class-1 entries are "near" (every
+// term), the rest "far" (excluded volume and Debye-Hueckel at most). (2) The workgroup's
+// near entries (every particle's class-0 entries, then every particle's class-1
+// entries), its bond slots and its far entries, laid end to end with each part starting
+// on a wavefront boundary, are evaluated one per thread, 256 per round, so the base-base,
 // coaxial and bonded paths run on full wavefronts instead of on the few lanes of each
 // particle that hold such a pair; each evaluation leaves its force, torque and energy in
 // an LDS record. (3) Each particle's lanes sum its records in a fixed order (runs are
