@@ -301,7 +301,8 @@ __global__ void k_step2(StepArgs a) {
 // term), the rest "far" (excluded volume and Debye-Hueckel at most). (2) The workgroup's
 // near entries (every particle's class-0 entries, then every particle's class-1
 // entries), its bond slots and its far entries, laid end to end with each part starting
-// on a wavefront boundary, are evaluated one per thread, 256 per round, so the base-base,
+// on a wavefront boundary, are evaluated one per thread (far entries two per thread on
+// grids too small to fill the GPU: fewer rounds), 256 per round, so the base-base,
 // coaxial and bonded paths run on full wavefronts instead of on the few lanes of each
 // particle that hold such a pair; each evaluation leaves its force, torque and energy in
 // an LDS record. (3) Each particle's lanes sum its records in a fixed order (runs are
@@ -329,7 +330,7 @@ __device__ __forceinline__ uint32_t owner_of(const uint32_t* off, uint32_t x) {
 // spills: large systems are bound by issue rate across many waves); the fused one, used
 // only while the grid holds <= 2 waves per SIMD, and the single-point energy + virial
 // one, unconstrained.
-template<int P, bool WANT_E, bool WANT_V, bool COUNT, bool FUSED = false>
+template<int P, bool WANT_E, bool WANT_V, bool COUNT, bool FUSED = false, bool FAR2 = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((FUSED || (WANT_E && WANT_V)) ? 1 : 3)))
 void k_forces(ForceArgs a) {
     constexpr uint32_t TPP = 256 / P;
@@ -343,6 +344,7 @@ void k_forces(ForceArgs a) {
     __shared__ size_t s_head[P];
     __shared__ uint32_t s_nnear[P];
     __shared__ uint32_t s_cnt[P];  // rebuild launches: each particle's row length so far
+    __shared__ uint32_t s_nn[P];   // row lengths
     __shared__ uint32_t s_off[3][P + 1];  // exclusive prefix sums of class-0 near / class-1 near / far counts
     __shared__ float s_rec[NREC][256];
     // The term parameters, staged in LDS: the inlined term evaluators reach them through
@@ -360,6 +362,7 @@ void k_forces(ForceArgs a) {
     const DParams& P_ = *(const DParams*)s_par;
     const uint32_t mask = a.mask;
     const NbrArgs& nb = a.nb;
+    constexpr uint32_t fpk = FAR2 ? 2u : 1u;  // far row entries per thread
     const bool rebuild = nb.force_rebuild || (nb.rebuild_flag && *nb.rebuild_flag);
 
     double4 pi = make_double4(0, 0, 0, 0);
@@ -456,7 +459,8 @@ void k_forces(ForceArgs a) {
         s_nnear[slot] = active ? nnear : 0u;
         s_off[0][slot + 1] = active ? nA : 0u;
         s_off[1][slot + 1] = active ? nnear - nA : 0u;
-        s_off[2][slot + 1] = active ? nn - nnear : 0u;
+        s_off[2][slot + 1] = active ? (nn - nnear + fpk - 1u) / fpk : 0u;  // far slots
+        s_nn[slot] = active ? nn : 0u;
         if (active) {
             s_pos[slot] = pi;
             s_fr[0][slot] = ld_f0; s_fr[1][slot] = ld_f1; s_fr[2][slot] = ld_f2;
@@ -501,24 +505,25 @@ void k_forces(ForceArgs a) {
         float rec[NREC];
 #pragma unroll
         for (int k = 0; k < NREC; ++k) rec[k] = 0.f;
-        bool pair = false;
+        uint32_t npair = 0;  // row entries this thread evaluates: one near, or up to fpk far
         uint32_t s = 0, k = 0;
         if (e < NA) {
             s = owner_of<P>(s_off[0], e);
             k = e - s_off[0][s];
-            pair = true;
+            npair = 1;
         } else if (e < NN) {
             const uint32_t x = e - NA;
             s = owner_of<P>(s_off[1], x);
             k = (s_off[0][s + 1] - s_off[0][s]) + (x - s_off[1][s]);
-            pair = true;
+            npair = 1;
         } else if (e >= F0 && e < NT) {
             s = owner_of<P>(s_off[2], e - F0);
-            k = s_nnear[s] + (e - F0 - s_off[2][s]);
-            pair = true;
+            k = s_nnear[s] + fpk * (e - F0 - s_off[2][s]);
+            npair = (fpk == 2u && k + 1u < s_nn[s]) ? 2u : 1u;
         }
-        if (pair) {
-            const uint32_t j = (srow ? lds_rows[s_head[s] + k] : a.nlist[s_head[s] + k]) & kIndexMask;
+        // one row entry's evaluation into this thread's record (add: onto what it holds)
+        auto entry = [&](uint32_t kk, bool add) {
+            const uint32_t j = (srow ? lds_rows[s_head[s] + kk] : a.nlist[s_head[s] + kk]) & kIndexMask;
             const double4 ps = s_pos[s];
             const float4 f0 = s_fr[0][s], f1 = s_fr[1][s], f2 = s_fr[2][s];
             const double4 pj = a.pos[j];
@@ -532,14 +537,26 @@ void k_forces(ForceArgs a) {
             const float rsq = dot(dr, dr);
             V Fp = zero3(), Tp = zero3();
             const float ep = pair_terms<COUNT>(P_, mask, tp, rsq, -dr, fi, fj, f0.w * g0.w, Fp, Tp, et, cnt);
-            rec[0] = Fp.x; rec[1] = Fp.y; rec[2] = Fp.z;
-            rec[3] = Tp.x; rec[4] = Tp.y; rec[5] = Tp.z;
-            if constexpr (NBASE > 6) rec[6] = 0.5f * ep;
+            float x[NREC];
+#pragma unroll
+            for (int q = 0; q < NREC; ++q) x[q] = 0.f;
+            x[0] = Fp.x; x[1] = Fp.y; x[2] = Fp.z;
+            x[3] = Tp.x; x[4] = Tp.y; x[5] = Tp.z;
+            if constexpr (NBASE > 6) x[6] = 0.5f * ep;
             if (WANT_V) {
-                rec[RV + 0] = 0.5f * dr.x * Fp.x; rec[RV + 1] = 0.5f * dr.x * Fp.y; rec[RV + 2] = 0.5f * dr.x * Fp.z;
-                rec[RV + 3] = 0.5f * dr.y * Fp.y; rec[RV + 4] = 0.5f * dr.y * Fp.z; rec[RV + 5] = 0.5f * dr.z * Fp.z;
+                x[RV + 0] = 0.5f * dr.x * Fp.x; x[RV + 1] = 0.5f * dr.x * Fp.y; x[RV + 2] = 0.5f * dr.x * Fp.z;
+                x[RV + 3] = 0.5f * dr.y * Fp.y; x[RV + 4] = 0.5f * dr.y * Fp.z; x[RV + 5] = 0.5f * dr.z * Fp.z;
             }
-        } else if (e >= B0 && e < B0 + 2u * P && NB) {
+#pragma unroll
+            for (int q = 0; q < NREC; ++q) rec[q] = add ? rec[q] + x[q] : x[q];
+        };
+        if constexpr (FAR2) {
+            // up to two far entries of one particle
+            for (uint32_t h = 0; h < npair; ++h) entry(k + h, h > 0);
+        } else {
+            if (npair) entry(k, false);
+        }
+        if (npair == 0 && e >= B0 && e < B0 + 2u * P && NB) {
             // bond slot (e - B0) & 1 of particle (e - B0) / 2
             s = (e - B0) >> 1;
             const uint32_t ib = xcd_block() * P + s;
@@ -1009,15 +1026,15 @@ void upload_params(oxdna_ctx* c) {
 // ------------------------------------------------------------------ force launches
 // One templated launcher for every instantiation: TPP lanes per particle (8, 16, 32).
 void check_force_args(const ForceArgs& a);
-template<bool E, bool V, bool CNT, bool FUSED = false>
+template<bool E, bool V, bool CNT, bool FUSED = false, bool FAR2 = false>
 void launch_k(int tpp, dim3 g, dim3 b, size_t lds, hipStream_t st, const ForceArgs& a) {
     check_force_args(a);
     // tpp lanes per particle: 256 / tpp particles per workgroup
     switch (tpp) {
-    case 4: hipLaunchKernelGGL((k_forces<64, E, V, CNT, FUSED>), g, b, lds, st, a); break;
-    case 8: hipLaunchKernelGGL((k_forces<32, E, V, CNT, FUSED>), g, b, lds, st, a); break;
-    case 16: hipLaunchKernelGGL((k_forces<16, E, V, CNT, FUSED>), g, b, lds, st, a); break;
-    default: hipLaunchKernelGGL((k_forces<8, E, V, CNT, FUSED>), g, b, lds, st, a); break;
+    case 4: hipLaunchKernelGGL((k_forces<64, E, V, CNT, FUSED, FAR2>), g, b, lds, st, a); break;
+    case 8: hipLaunchKernelGGL((k_forces<32, E, V, CNT, FUSED, FAR2>), g, b, lds, st, a); break;
+    case 16: hipLaunchKernelGGL((k_forces<16, E, V, CNT, FUSED, FAR2>), g, b, lds, st, a); break;
+    default: hipLaunchKernelGGL((k_forces<8, E, V, CNT, FUSED, FAR2>), g, b, lds, st, a); break;
     }
 }
 void check_force_args(const ForceArgs& a) {
@@ -1041,10 +1058,12 @@ void launch_k_any(int tpp, bool e, bool v, dim3 g, dim3 b, size_t lds, hipStream
     check_force_args(a);
     if (a.integrate == 2) {
         if (e || v || a.accumulate) throw Fail("internal: fused run step with energies, virial or accumulation");
-        launch_k<false, false, false, true>(tpp, g, b, lds, st, a);
+        // a run step is fused only on grids too small to fill the GPU: far entries in pairs
+        launch_k<false, false, false, true, true>(tpp, g, b, lds, st, a);
     } else if (e && v) launch_k<true, true, false>(tpp, g, b, lds, st, a);
     else if (e) launch_k<true, false, false>(tpp, g, b, lds, st, a);
     else if (v) launch_k<false, true, false>(tpp, g, b, lds, st, a);
+    else if (a.far_pairs) launch_k<false, false, false, false, true>(tpp, g, b, lds, st, a);
     else launch_k<false, false, false>(tpp, g, b, lds, st, a);
 }
 
@@ -1138,6 +1157,7 @@ void ensure_ready(oxdna_ctx* c) {
 }
 
 StepArgs step_args(oxdna_ctx* c, bool with_cells, uint64_t t);
+bool fuse_second_half(const oxdna_ctx* c);
 
 // Forces of run step k (k = 0: the stored state). rebuild: 0 never (rows are current),
 // 1 always, 2 when step k's skin test asked for it (run steps, which also clear step
@@ -1153,6 +1173,7 @@ ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild, u
     a.n_near = c->n_near.p;
     a.nlist = c->nlist.p;
     a.head = c->head.p;
+    a.far_pairs = fuse_second_half(c) ? 1 : 0;
     a.bpart = c->bpart.p;
     a.binfo = c->binfo.p;
     a.P = c->dparams.p;

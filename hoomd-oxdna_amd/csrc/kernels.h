@@ -268,6 +268,9 @@ struct ForceArgs {
     unsigned long long* counts;  // COUNT instantiation: per term (inside cut-off, nonzero)
     NbrArgs nb;
     uint32_t* rebuild_clear;  // run steps: the request of step k + 2, cleared for step k + 1 to set
+    // 1: each thread evaluates two far row entries of a particle (latency-bound grids:
+    // fewer rounds); 0: one (grids that fill the GPU: fewer instructions per round)
+    int far_pairs;
     // 0: forces only; 1: then the Langevin second half-kick of step k (st.step_in); 2: then
     // also the first half of step k + 1 (st), for the particle's own lane
     int integrate;
