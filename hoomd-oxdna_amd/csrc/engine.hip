@@ -309,6 +309,35 @@ __global__ void k_step2(StepArgs a) {
 // bit-reproducible), then store, or (FUSED) integrate.
 constexpr int kRecBase = 7;  // F (3), T (3), half energy
 
+// Sum over the TPP adjacent lanes of a particle, left in every one of them: DPP moves
+// within a row of 16 lanes (rotations for 16 lanes; half-row mirror and quad
+// permutations for 8 and 4) where the lanes of a particle fit one row, lane shuffles
+// otherwise. The order of the additions is fixed.
+template<int CTRL>
+__device__ __forceinline__ float dpp_mov(float x) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+template<int TPP>
+__device__ __forceinline__ float lane_sum(float x) {
+    if constexpr (TPP == 16) {
+        x += dpp_mov<0x128>(x);  // row_ror:8
+        x += dpp_mov<0x124>(x);  // row_ror:4
+        x += dpp_mov<0x122>(x);  // row_ror:2
+        x += dpp_mov<0x121>(x);  // row_ror:1
+    } else if constexpr (TPP == 8) {
+        x += dpp_mov<0x141>(x);  // row_half_mirror: lane l of 8 takes 7 - l
+        x += dpp_mov<0x4E>(x);   // quad_perm [2, 3, 0, 1]
+        x += dpp_mov<0xB1>(x);   // quad_perm [1, 0, 3, 2]
+    } else if constexpr (TPP == 4) {
+        x += dpp_mov<0x4E>(x);
+        x += dpp_mov<0xB1>(x);
+    } else {
+#pragma unroll
+        for (uint32_t m = TPP / 2; m > 0; m >>= 1) x += __shfl_xor(x, m);
+    }
+    return x;
+}
+
 // Largest s in [0, P) with off[s] <= x (off ascending, off[0] = 0).
 template<int P>
 __device__ __forceinline__ uint32_t owner_of(const uint32_t* off, uint32_t x) {
@@ -469,18 +498,17 @@ void k_forces(ForceArgs a) {
         }
     }
     __syncthreads();
-    if (t < 64) {  // exclusive prefix sums over the P particles (P <= 64: one wavefront)
+    if (t < 3) {  // exclusive prefix sums over the P particles, one thread per count
+        uint32_t v[P];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            uint32_t v = t < P ? s_off[c][t + 1] : 0u;
+        for (int q = 0; q < P; ++q) v[q] = s_off[t][q + 1];
+        uint32_t sum = 0;
 #pragma unroll
-            for (uint32_t d = 1; d < 64; d <<= 1) {
-                uint32_t y = __shfl_up(v, d, 64);
-                if (t >= d) v += y;
-            }
-            if (t < P) s_off[c][t + 1] = v;
-            if (t == 0) s_off[c][0] = 0u;
+        for (int q = 0; q < P; ++q) {
+            sum += v[q];
+            s_off[t][q + 1] = sum;
         }
+        s_off[t][0] = 0u;
     }
     __syncthreads();
     // near entries: every particle's class-0 entries, then every particle's class-1
@@ -622,10 +650,7 @@ void k_forces(ForceArgs a) {
     }
     // reduce the TPP lanes of a particle (adjacent lanes of one wavefront)
 #pragma unroll
-    for (uint32_t m = TPP / 2; m > 0; m >>= 1) {
-#pragma unroll
-        for (int q = 0; q < NREC; ++q) acc[q] += __shfl_xor(acc[q], m);
-    }
+    for (int q = 0; q < NREC; ++q) acc[q] = lane_sum<TPP>(acc[q]);
     if (WANT_E && a.split && active && lane == 0)
         a.split[i] = make_double4(0.5 * acc[RS + 0], 0.5 * acc[RS + 1], 0.5 * acc[RS + 2], 0.0);
     if (active && lane == 0) {
