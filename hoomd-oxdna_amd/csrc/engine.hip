@@ -410,13 +410,22 @@ void k_forces(ForceArgs a) {
     }
     // FUSED: the particle's integrator state, and the part of the merged half-kicks that
     // needs no force (noise, drag), while the first loads are in flight
-    double4 e_v, e_p, e_q, e_plast;
-    if (FUSED && active && lane == 0) {
+    // FUSED: each particle's integrator state with the force-free part of the merged
+    // half-kicks (noise, drag) applied, computed during the first round by a wavefront
+    // whose entries there are light (the class-0 entries lead), kept in LDS for the
+    // epilogue
+    __shared__ double4 s_st[FUSED ? 4 : 1][FUSED ? P : 1];  // v, p, q, position of the last rebuild
+    auto pre_particle = [&](uint32_t ps) {
+        const uint32_t ip = xcd_block() * P + ps;
+        if (ip >= a.N) return;
         const StepArgs& s = a.st;
-        e_v = s.vel[i]; e_p = s.angmom[i]; e_q = s.quat[i]; e_plast = s.pos_last[i];
-        step_clear(s, s.pos_out[i]);
-        lv_merge_pre(s.lv, i, *s.step_in, s.seed, e_q, e_v, e_p);
-    }
+        double4 v = s.vel[ip], p = s.angmom[ip];
+        const double4 q = s.quat[ip];
+        s_st[3][ps] = s.pos_last[ip];
+        step_clear(s, s.pos_out[ip]);
+        lv_merge_pre(s.lv, ip, *s.step_in, s.seed, q, v, p);
+        s_st[0][ps] = v; s_st[1][ps] = p; s_st[2][ps] = q;
+    };
 
     // ---- (1) rows
     uint32_t nn = 0, nnear = 0, nA = 0;  // row length, near entries, of them class 0
@@ -528,6 +537,12 @@ void k_forces(ForceArgs a) {
 #pragma unroll
     for (int k = 0; k < NREC; ++k) acc[k] = 0.f;
     const int nt = P_.ntypes;
+    // the wavefront after the class-0 entries does the integrator's force-free part
+    const uint32_t w_pre = min(3u, (NA + 63u) / 64u);
+    if (FUSED && NT == 0) {
+        if (t < P) pre_particle(t);
+        __syncthreads();
+    }
     for (uint32_t r0 = 0; r0 < NT; r0 += 256) {
         const uint32_t e = r0 + t;
         float rec[NREC];
@@ -625,6 +640,7 @@ void k_forces(ForceArgs a) {
         }
 #pragma unroll
         for (int q = 0; q < NREC; ++q) s_rec[q][t] = rec[q];
+        if (FUSED && r0 == 0 && t / 64u == w_pre && t % 64u < P) pre_particle(t % 64u);
         __syncthreads();
         // this particle's records of the round: near entries, bond slots, far entries
         if (active) {
@@ -663,7 +679,8 @@ void k_forces(ForceArgs a) {
             // first half of step k + 1: both touch only particle i's own state, and the
             // positions and frames of step k + 1 go to the other buffer of each pair, so no
             // workgroup still reading step k sees them
-            merged_particle(a.st, i, pi, e_v, e_q, e_p, fo, to, rebuild ? pi : e_plast, charge_i);
+            merged_particle(a.st, i, pi, s_st[0][slot], s_st[2][slot], s_st[1][slot], fo, to,
+                            rebuild ? pi : s_st[3][slot], charge_i);
         } else {
             if (a.accumulate) {
                 float4 f = a.force[i], tq = a.torque[i];
