@@ -537,6 +537,18 @@ void k_forces(ForceArgs a) {
 #pragma unroll
     for (int k = 0; k < NREC; ++k) acc[k] = 0.f;
     const int nt = P_.ntypes;
+    // this thread's bond slot in the round that starts the bond slots: its partner and
+    // role loaded before the rounds begin (the bond wavefront's first dependent load)
+    const uint32_t rb0 = B0 & ~255u, eb = rb0 + t;
+    const bool bpre = NB && eb >= B0 && eb < B0 + 2u * P;
+    int b_other = -1, b_info = 0;
+    if (bpre) {
+        const uint32_t ib = xcd_block() * P + ((eb - B0) >> 1);
+        if (ib < a.N) {
+            b_other = a.bpart[2 * ib + ((eb - B0) & 1u)];
+            b_info = a.binfo[2 * ib + ((eb - B0) & 1u)];
+        }
+    }
     // the wavefront after the class-0 entries does the integrator's force-free part
     const uint32_t w_pre = min(3u, (NA + 63u) / 64u);
     if (FUSED && NT == 0) {
@@ -603,9 +615,10 @@ void k_forces(ForceArgs a) {
             // bond slot (e - B0) & 1 of particle (e - B0) / 2
             s = (e - B0) >> 1;
             const uint32_t ib = xcd_block() * P + s;
-            const int other = ib < a.N ? a.bpart[2 * ib + ((e - B0) & 1u)] : -1;
+            const bool pre = bpre && r0 == rb0;
+            const int other = pre ? b_other : (ib < a.N ? a.bpart[2 * ib + ((e - B0) & 1u)] : -1);
             if (other >= 0) {
-                const int info = a.binfo[2 * ib + ((e - B0) & 1u)];
+                const int info = pre ? b_info : a.binfo[2 * ib + ((e - B0) & 1u)];
                 const int role = info & 1;
                 const int bt = info >> 1;
                 const uint32_t A = role == 0 ? ib : (uint32_t)other;
