@@ -129,6 +129,25 @@ def test_particles_per_workgroup_variants(lanes):
     check_against_oracle(system, forces, f"gas dna2 fused, {lanes} lanes", lanes=lanes)
 
 
+def test_large_grid_paths():
+    """A grid that fills the GPU (4800 nt at 32 lanes per particle: more than 2048
+    wavefronts) takes the other kernel layout (one far row entry per thread) and, in a
+    run, a force launch plus the separate integrator kernel per step: forces against the
+    oracle before and after a 300-step Langevin run on that path."""
+    system = H.jiggle(systems.duplex_gas(200, 12, box=100.0, seed=21), seed=4, dx=0.01, dq=0.02)
+    assert system.n * 32 // 64 > 2048
+    forces, _ = dna2.forces(fused=True)
+    check_against_oracle(system, forces, "large grid, 32 lanes", lanes=32)
+    s = _langevin_sim(system, forces, seed=3, lanes=32)
+    s.run(300)
+    snap = s.state.get_snapshot()
+    assert np.isfinite(snap.particles.position).all()
+    assert forces[0].fene_energy / system.n < 1.0  # no backbone bond broke
+    hot = systems.System(snap.particles.position, snap.particles.orientation, system.typeid, system.bonds,
+                         system.box, system.strand, system.bases, system.n3, system.n5, system.charge)
+    check_against_oracle(hot, dna2.forces(fused=True)[0], "large grid after 300 steps, 32 lanes", lanes=32)
+
+
 def test_pinned_lanes_reproducible():
     """Same lanes per particle, same seed: two runs give bit-identical trajectories."""
     system = H.cfg1_system(2)
