@@ -353,9 +353,10 @@ __device__ __forceinline__ uint32_t owner_of(const uint32_t* off, uint32_t x) {
 }
 
 // FUSED: a run step whose epilogue also takes the particle through the Langevin second
-// half of step k and the first half of step k + 1 (a.integrate == 2); its loads are
-// issued before the pair loop, and the force and torque, consumed in the epilogue, are
-// not stored. Register budget: the plain kernel at 3 waves per SIMD (168 VGPRs, no
+// half of step k and the first half of step k + 1 (a.integrate == 2); the force-free part
+// of that runs during the first round, and the force and torque, consumed in the
+// epilogue, are not stored. FAR2: a thread evaluates two far row entries of a particle
+// (grids too small to fill the GPU: fewer rounds). Register budget: the plain kernel at 3 waves per SIMD (168 VGPRs, no
 // spills: large systems are bound by issue rate across many waves); the fused one, used
 // only while the grid holds <= 2 waves per SIMD, and the single-point energy + virial
 // one, unconstrained.
