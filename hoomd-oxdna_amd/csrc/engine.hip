@@ -313,7 +313,7 @@ constexpr int kRecBase = 7;  // F (3), T (3), half energy
 // within a row of 16 lanes (rotations for 16 lanes; half-row mirror and quad
 // permutations for 8 and 4) where the lanes of a particle fit one row, lane shuffles
 // otherwise. The order of the additions is fixed. Called with every lane of the
-// wavefront active (DPP reads its source lanes whatever their EXEC bit).
+// wavefront active: a DPP move from an inactive lane does not deliver its value.
 template<int CTRL>
 __device__ __forceinline__ float dpp_mov(float x) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
