@@ -179,6 +179,7 @@ __device__ __forceinline__ void row_candidates(const NbrArgs& nb, const double4&
             n -= h;
         }
         const uint32_t j = nb.members[(size_t)c_id[lo] * nb.cap + (t - c_st[lo])];
+        if (j == kHole) continue;  // left by a mispredicted binning
         const int cls = ref_pair<TPP>(nb, pi, bsi, i, j, b0, b1);
         if (cls != 0) {
             const uint32_t p = atomicAdd(counter, 1u);
@@ -248,6 +249,39 @@ __device__ __forceinline__ void merged_particle(const StepArgs& a, uint32_t i, d
     a.pos_out[i] = x;
     a.vel[i] = v;
     const CellSlot b = step_bin(a, x, plast);
+    lv_rotate(a.lv, q, p);
+    step_finish(a, i, x, q, p, b, charge);
+}
+
+// merged_particle for the fused run step, whose particle already holds a cell slot bp
+// (taken during the rounds for the position predicted before the force kick): the slot
+// takes the particle if the new position is in that cell; otherwise it becomes a hole,
+// the particle is binned again, and the predicted cell goes to a.cells.rebin for its
+// clearing two steps later.
+__device__ __forceinline__ void merged_particle_pred(const StepArgs& a, uint32_t i, double4 x, double4 v,
+                                                     const double4& q0, double4 p, const float4& f, const float4& tq,
+                                                     const double4& plast, float charge, CellSlot bp) {
+    double4 q = q0;
+    lv_merge_kick(a.lv, f.x, f.y, f.z, tq.x, tq.y, tq.z, q, v, p);
+    lv_drift(a.lv, a.L, a.Linv, v, x);
+    a.pos_out[i] = x;
+    a.vel[i] = v;
+    CellSlot b = bp;
+    if (a.cells.count) {
+        const uint32_t c = cell_index(x.x, x.y, x.z, a.cells.Linv, a.cells.nc);
+        uint32_t re = kNoCell;
+        if (c != bp.cell) {
+            if (bp.slot < a.cells.cap) a.cells.members[(size_t)bp.cell * a.cells.cap + bp.slot] = kHole;
+            b.cell = c;
+            b.slot = atomicAdd(&a.cells.count[c], 1u);
+            re = bp.cell;
+        }
+        a.cells.rebin[i] = re;
+    }
+    const double dx = min_image(x.x - plast.x, a.L[0], a.Linv[0]);
+    const double dy = min_image(x.y - plast.y, a.L[1], a.Linv[1]);
+    const double dz = min_image(x.z - plast.z, a.L[2], a.Linv[2]);
+    if (dx * dx + dy * dy + dz * dz > a.skin_half2) atomicOr(a.rebuild_set, 1u);
     lv_rotate(a.lv, q, p);
     step_finish(a, i, x, q, p, b, charge);
 }
@@ -417,6 +451,7 @@ void k_forces(ForceArgs a) {
     // whose entries there are light (the class-0 entries lead), kept in LDS for the
     // epilogue
     __shared__ double4 s_st[FUSED ? 4 : 1][FUSED ? P : 1];  // v, p, q, position of the last rebuild
+    __shared__ uint32_t s_bin[2][FUSED ? P : 1];            // predicted cell and slot
     auto pre_particle = [&](uint32_t ps) {
         const uint32_t ip = xcd_block() * P + ps;
         if (ip >= a.N) return;
@@ -425,8 +460,22 @@ void k_forces(ForceArgs a) {
         const double4 q = s.quat[ip];
         s_st[3][ps] = s.pos_last[ip];
         step_clear(s, s.pos_out[ip]);
+        if (s.cells.count_prev) {
+            const uint32_t rc = s.cells.rebin[ip];  // a mispredicted cell of two steps back
+            if (rc < s.cells.ncell) s.cells.count_prev[rc] = 0u;
+        }
         lv_merge_pre(s.lv, ip, *s.step_in, s.seed, q, v, p);
         s_st[0][ps] = v; s_st[1][ps] = p; s_st[2][ps] = q;
+        // the cell slot for the position the drift gives before the force kick (the kick
+        // moves it by dt^2 F / m): its atomic's round trip overlaps the rounds
+        uint32_t pc = 0u, pslot = 0u;
+        if (s.cells.count) {
+            double4 xp = a.pos[ip];
+            lv_drift(s.lv, s.L, s.Linv, v, xp);
+            pc = cell_index(xp.x, xp.y, xp.z, s.cells.Linv, s.cells.nc);
+            pslot = atomicAdd(&s.cells.count[pc], 1u);
+        }
+        s_bin[0][ps] = pc; s_bin[1][ps] = pslot;
     };
 
     // ---- (1) rows
@@ -694,8 +743,8 @@ void k_forces(ForceArgs a) {
             // first half of step k + 1: both touch only particle i's own state, and the
             // positions and frames of step k + 1 go to the other buffer of each pair, so no
             // workgroup still reading step k sees them
-            merged_particle(a.st, i, pi, s_st[0][slot], s_st[2][slot], s_st[1][slot], fo, to,
-                            rebuild ? pi : s_st[3][slot], charge_i);
+            merged_particle_pred(a.st, i, pi, s_st[0][slot], s_st[2][slot], s_st[1][slot], fo, to,
+                                 rebuild ? pi : s_st[3][slot], charge_i, CellSlot{s_bin[0][slot], s_bin[1][slot]});
         } else {
             if (a.accumulate) {
                 float4 f = a.force[i], tq = a.torque[i];
@@ -817,6 +866,7 @@ struct oxdna_ctx {
     DBuf<double4> pos, vel, quat, angmom, pos_last, bd, bd_tq;
     DBuf<double> charge;
     DBuf<float4> fr0, fr1, fr2, force, torque;
+    DBuf<uint32_t> rebin;  // 2 per particle (step parity): see CellArgs::rebin
     DBuf<float> virial;
     DBuf<int> bpart, binfo;
     DBuf<uint32_t> n_neigh, n_near, nlist, cell_count, cell_members;
@@ -871,6 +921,7 @@ struct oxdna_ctx {
         pos.release(); vel.release(); quat.release(); angmom.release(); pos_last.release();
         bd.release(); bd_tq.release();
         charge.release(); fr0.release(); fr1.release(); fr2.release(); force.release(); torque.release();
+        rebin.release();
         virial.release(); bpart.release(); binfo.release(); n_neigh.release(); n_near.release(); nlist.release();
         cell_count.release(); cell_members.release(); head.release(); epart.release(); energy.release();
         flags.release(); dparams.release();
@@ -1188,6 +1239,8 @@ CellArgs cell_args(oxdna_ctx* c, bool with_cells, uint64_t t) {
     a.count_prev = with_cells ? c->cell_count.p + (size_t)((t + 1) % 3) * nct : nullptr;
     a.members = c->cell_members.p + (size_t)(t & 1) * nct * c->cell_cap;
     a.flags = c->flags.p;
+    a.ncell = (uint32_t)nct;
+    a.rebin = c->rebin.p + (size_t)(t & 1) * c->N;
     return a;
 }
 
@@ -1813,6 +1866,7 @@ int oxdna_set_state(oxdna_ctx* c, unsigned int n, const double* pos, const doubl
             c->pos.release(); c->quat.release(); c->vel.release(); c->angmom.release(); c->pos_last.release();
             c->bd.release(); c->bd_tq.release();
             c->charge.release(); c->fr0.release(); c->fr1.release(); c->fr2.release(); c->force.release();
+            c->rebin.release();
             c->torque.release(); c->virial.release(); c->n_neigh.release(); c->n_near.release(); c->nlist.release();
             c->head.release();
             c->epart.release();
@@ -1823,6 +1877,8 @@ int oxdna_set_state(oxdna_ctx* c, unsigned int n, const double* pos, const doubl
         c->pos.alloc(2 * (size_t)n); c->quat.alloc(n); c->vel.alloc(n); c->angmom.alloc(n); c->pos_last.alloc(n);
         c->bd.alloc(n); c->bd_tq.alloc(n);
         c->charge.alloc(n); c->fr0.alloc(2 * (size_t)n); c->fr1.alloc(2 * (size_t)n); c->fr2.alloc(2 * (size_t)n);
+        c->rebin.alloc(2 * (size_t)n);
+        HIP_OK(hipMemsetAsync(c->rebin.p, 0xff, sizeof(uint32_t) * 2 * n, c->stream));  // kNoCell
         c->force.alloc(n); c->torque.alloc(n); c->virial.alloc(6 * (size_t)n); c->n_neigh.alloc(n);
         c->n_near.alloc(n);
         c->epart.alloc((size_t)blocks_for((size_t)n * kTPPMax) * 8);

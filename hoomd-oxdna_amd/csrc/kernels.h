@@ -157,7 +157,15 @@ struct CellArgs {
     uint32_t* count_prev;  // step kernels: the counts of two steps back (zeroed as they go)
     uint32_t* members;
     DevFlags* flags;
+    uint32_t ncell;     // cells of one count buffer
+    // fused run steps bin a particle by its position predicted before the force kick:
+    // per particle, the predicted cell of a misprediction two steps back (its count in
+    // count_prev is cleared too) or kNoCell
+    uint32_t* rebin;
 };
+
+constexpr uint32_t kNoCell = 0xffffffffu;
+constexpr uint32_t kHole = 0xffffffffu;  // a cell member slot a misprediction left empty
 
 // Bins particle i; returns the occupancy it saw (slot + 1) for the caller's
 // wave-reduced update of flags->max_cell.

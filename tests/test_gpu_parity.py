@@ -148,6 +148,34 @@ def test_large_grid_paths():
     check_against_oracle(hot, dna2.forces(fused=True)[0], "large grid after 300 steps, 32 lanes", lanes=32)
 
 
+def test_rows_cover_every_pair_after_a_hot_run():
+    """A hot Langevin run of a duplex gas (many in-kernel row rebuilds and cell crossings:
+    the run step bins each particle by its position before the force kick, so now and
+    then a cell holds a hole) ends with rows that hold every pair closer than the
+    largest cut-off (less a margin for the last step's motion) at the current positions."""
+    system = H.jiggle(systems.duplex_gas(40, 12, box=30.0, seed=31), seed=5, dx=0.01, dq=0.02)
+    forces, nl = dna2.forces(fused=True)
+    s = _langevin_sim(system, forces, seed=11, kT=0.2)
+    s.run(3000)
+    st = s.stats()
+    pos = s.state.get_snapshot().particles.position
+    got = {tuple(p) for p in s.neighbor_pairs()}
+    rcut = max(float(f.r_cut[("A", "A")]) for f in forces[1:])
+    L = np.asarray(system.box, float)[:3]
+    bonded = {tuple(sorted(b)) for b in np.asarray(system.bonds)}
+    missing = []
+    for i in range(system.n):
+        d = pos[i + 1:] - pos[i]
+        d -= L * np.round(d / L)
+        for k in np.nonzero((d * d).sum(1) < (rcut - 0.05) ** 2)[0]:
+            pair = (i, i + 1 + int(k))
+            if pair not in bonded and pair not in got:
+                missing.append(pair)
+    _say(f"hot run: {st['nlist_builds']} row builds, {len(got)} row pairs, missing {len(missing)}")
+    assert st["nlist_builds"] > 5
+    assert not missing
+
+
 def test_pinned_lanes_reproducible():
     """Same lanes per particle, same seed: two runs give bit-identical trajectories."""
     system = H.cfg1_system(2)
