@@ -21,9 +21,11 @@
 //     (the reference's fd_check method, pytest/_helpers.py:137-181), Newton's third
 //     law, total angular momentum, global rotation / translation invariance;
 //   * neighbour pair sets: brute-force O(N^2) enumeration vs the cell-list builder.
+//   * the f4Dsin small-angle fallback: equal to the limit of the inner branch as
+//     sin(theta) -> 0 (theta0 at 0 or pi).
 // Parity unpinned (no golden data exists offline): absolute agreement with oxDNA's
-// split_energy.dat, and reference-specific conventions that no invariant fixes
-// (documented where they occur).
+// split_energy.dat, and reference-specific conventions that no invariant fixes (the
+// coaxial theta1 fallback 2 sa, where the branch it replaces diverges).
 #pragma once
 
 #include <cmath>

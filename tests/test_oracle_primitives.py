@@ -117,6 +117,28 @@ def test_f4_angular_closed_form_and_c1(name):
         assert O.prim(2, th, p)[1] * math.sin(th) == pytest.approx(fd, rel=1e-5, abs=1e-5)
 
 
+def test_f4dsin_small_sine_fallback_is_the_limit():
+    """Where sin(theta)^2 <= 1e-8 the inner branch returns -m 2a instead of
+    -m 2a (theta - t0) / sin(theta) (reference src/OxDNANumerics.h:236-241). The inner
+    branch reaches sin(theta) -> 0 only for t0 at 0 or pi, and there the ratio tends to
+    -2a (t0 = 0, theta -> 0+) or +2a (t0 = pi, theta -> pi-): the fallback must equal
+    that limit, and the values just outside the guard must tend to it."""
+    sets = {n: list(p) for n, p in F4_SETS.items() if min(abs(p[2]), abs(p[2] - math.pi)) < 1e-9}
+    assert {"hb_t1", "hb_t4"} <= set(sets)
+    for name, p in sets.items():
+        a, b, t0, ts, tc = p
+        side = 1.0 if abs(t0) < 1e-9 else -1.0  # the side of t0 inside [0, pi]
+        limit = -2 * a if side > 0 else 2 * a
+        for eps in (1e-6, 5e-5, 9e-5):  # sin^2 <= 1e-8: the fallback
+            th = t0 + side * eps
+            assert math.sin(th) ** 2 <= 1e-8
+            assert O.prim(2, th, p)[1] == pytest.approx(limit, rel=1e-12), name
+        for eps in (1.01e-4, 3e-4, 1e-3):  # outside the guard: -m 2a eps / sin(eps)
+            th = t0 + side * eps
+            assert math.sin(th) ** 2 > 1e-8
+            assert O.prim(2, th, p)[1] == pytest.approx(limit, rel=2e-7), name
+
+
 @pytest.mark.parametrize("name", sorted(F5_SETS))
 def test_f5_dihedral_closed_form_and_c1(name):
     p = list(F5_SETS[name])
