@@ -83,6 +83,29 @@ def write_configuration(path, t, box, pos, quat, vel=None, angvel=None, energy=(
             fh.write(" ".join(f"{x:.15g}" for x in cols) + "\n")
 
 
+def body_angular_momentum(quat, angmom):
+    """L in the body frame from the quaternion momentum p of the engine's state
+    (HOOMD's convention: q* (x) p = (0, 2 L_body))."""
+    q = np.asarray(quat, dtype=float)
+    p = np.asarray(angmom, dtype=float)
+    qw, qv = q[..., :1], -q[..., 1:]  # q*
+    pw, pv = p[..., :1], p[..., 1:]
+    return 0.5 * (qw * pv + pw * qv + np.cross(qv, pv))
+
+
+def angular_velocity(quat, angmom, moment_inertia):
+    """Lab-frame angular velocity R(q) (L_body / I) per particle."""
+    return rotate(quat, body_angular_momentum(quat, angmom) / np.asarray(moment_inertia, dtype=float))
+
+
+def kinetic_energy(mass, velocity, quat, angmom, moment_inertia):
+    """Translational plus rotational kinetic energy of the whole system."""
+    lb = body_angular_momentum(quat, angmom)
+    m = np.asarray(mass, dtype=float)
+    v = np.asarray(velocity, dtype=float)
+    return float(0.5 * np.sum(m[:, None] * v * v) + 0.5 * np.sum(lb * lb / np.asarray(moment_inertia, dtype=float)))
+
+
 # ------------------------------------------------------------------- quaternions
 def rotate(q, v):
     """Rotate v (3,) or (n,3) by unit quaternions q (n,4) (w,x,y,z); returns (n,3)."""

@@ -24,6 +24,7 @@ import types as _types
 import numpy as np
 
 from . import _abi
+from . import io
 from . import _engine
 from .forces import AnisotropicPair, OxDNABonded
 
@@ -97,10 +98,43 @@ class Integrator:
             raise ValueError("one integration method (Langevin) per integrator")
 
 
+class ConfTrajectory:
+    """Writes an oxDNA configuration frame to `filename` at every timestep that run()
+    reaches with timestep % period == 0: the line `t = ...`, the box, `E = Etot U K`
+    (per nucleotide: potential energy of the attached force field, kinetic energy of
+    translation and rotation), then per particle the position (shifted into [0, L]),
+    a1, a3, velocity and lab-frame angular velocity. mode "w" starts the file at the
+    first frame, "a" appends to it. Frames read back with io.iter_conf_frames."""
+
+    def __init__(self, filename, period, mode="w"):
+        if int(period) < 1:
+            raise ValueError("period must be a positive number of steps")
+        if mode not in ("w", "a"):
+            raise ValueError('mode must be "w" or "a"')
+        self.filename = filename
+        self.period = int(period)
+        self._append = mode == "a"
+        self.frames = 0
+
+    def write(self, simulation):
+        snap = simulation.state.get_snapshot()
+        part = snap.particles
+        n = part.N
+        pot = float(sum(simulation.term_energies().values()))
+        kin = io.kinetic_energy(part.mass, part.velocity, part.orientation, part.angmom, part.moment_inertia)
+        omega = io.angular_velocity(part.orientation, part.angmom, part.moment_inertia)
+        io.write_configuration(self.filename, simulation.timestep, snap.configuration.box[:3], part.position,
+                               part.orientation, part.velocity, omega,
+                               energy=((pot + kin) / n, pot / n, kin / n), append=self._append)
+        self._append = True
+        self.frames += 1
+
+
 class _Operations:
     def __init__(self, sim):
         self._sim = sim
         self._integrator = None
+        self.writers = []  # ConfTrajectory instances, called as run() reaches their periods
 
     @property
     def integrator(self):
@@ -258,8 +292,19 @@ class Simulation:
         else:
             if not self.operations.integrator.methods:
                 raise RuntimeError("running steps needs an integration method (methods.Langevin)")
-            self._check(self._lib.oxdna_run(self._ctx, steps, self._mask, 1 if self.use_graph else 0), "run")
-            self.timestep += steps
+            left = steps
+            while left > 0:
+                # run up to the next timestep at which a writer is due
+                chunk = left
+                for w in self.operations.writers:
+                    chunk = min(chunk, w.period - self.timestep % w.period)
+                self._check(self._lib.oxdna_run(self._ctx, chunk, self._mask, 1 if self.use_graph else 0), "run")
+                self.timestep += chunk
+                left -= chunk
+                self._cache.clear()
+                for w in self.operations.writers:
+                    if self.timestep % w.period == 0:
+                        w.write(self)
         self._cache.clear()
 
     def synchronize(self):

@@ -350,6 +350,46 @@ def test_run_grows_rows_mid_chunk():
     check_against_oracle(after, dna2.forces(fused=True)[0], "after implosion")
 
 
+def test_conf_trajectory_frames(tmp_path):
+    """ConfTrajectory writes a frame at each multiple of its period that run() reaches;
+    the last frame holds the final state, its energy line the force field's potential
+    energy per nucleotide; the run split at the frames follows the unsplit run."""
+    from oxdna import io
+    system = H.cfg1_system(4)
+    path = tmp_path / "traj.conf"
+    forces, _ = dna2.forces(fused=True)
+    s = _langevin_sim(system, forces, seed=3, lanes=16)
+    w = sim.ConfTrajectory(str(path), 10)
+    s.operations.writers.append(w)
+    s.run(25)
+    s.run(5)
+    frames = list(io.iter_conf_frames(str(path)))
+    assert [f[0] for f in frames] == [10.0, 20.0, 30.0] and w.frames == 3
+    snap = s.state.get_snapshot()
+    t, box, rows = frames[-1]
+    L = np.asarray(snap.configuration.box[:3])
+    assert np.allclose(box, L)
+    part = snap.particles
+    assert np.allclose(rows[:, 0:3], part.position + L / 2, atol=1e-12)
+    assert np.allclose(rows[:, 3:6], io.rotate(part.orientation, [1.0, 0.0, 0.0]), atol=1e-12)
+    assert np.allclose(rows[:, 6:9], io.rotate(part.orientation, [0.0, 0.0, 1.0]), atol=1e-12)
+    assert np.allclose(rows[:, 9:12], part.velocity, atol=1e-12)
+    assert np.allclose(rows[:, 12:15], io.angular_velocity(part.orientation, part.angmom, part.moment_inertia),
+                       atol=1e-12)
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    e_line = [x for x in lines if x.startswith("E =")][-1].split("=")[1].split()
+    u = sum(s.term_energies().values()) / system.n
+    assert float(e_line[1]) == pytest.approx(u, rel=1e-9, abs=1e-12)
+    # the same 30 steps without writers
+    forces2, _ = dna2.forces(fused=True)
+    s2 = _langevin_sim(system, forces2, seed=3, lanes=16)
+    s2.run(30)
+    dx = np.abs(s2.state.get_snapshot().particles.position - part.position).max()
+    _say("trajectory run vs plain run: max |dx|", dx)
+    assert dx < 1e-6
+
+
 def _rot_ke(snap):
     """Rotational kinetic energy per particle, I = 1: L_body = 1/2 (conj(q) p).v."""
     q, p = snap.particles.orientation, snap.particles.angmom

@@ -101,3 +101,34 @@ def test_top_conf_round_trip(tmp_path):
     assert [tuple(b) for b in snap.bonds.group] == [tuple(b) for b in s.bonds]
     t = io.read_topology(str(top))
     assert "".join(t["base"]) == "".join(s.bases)
+
+
+def test_angular_velocity_and_kinetic_energy_closed_form():
+    """A particle rotated by phi about z whose body-frame angular momentum is L: the
+    quaternion momentum is p = q (x) (0, 2 L) (q* (x) p = (0, 2 L_body)), the lab-frame
+    angular velocity R_z(phi) (L / I) and the rotational energy sum L^2 / (2 I)."""
+    from oxdna import io
+    phi = 0.7
+    q = np.array([[np.cos(phi / 2), 0.0, 0.0, np.sin(phi / 2)]])
+    L = np.array([0.3, -0.2, 0.5])
+    inertia = np.array([[2.0, 1.0, 4.0]])
+    qw, qv = q[0, 0], q[0, 1:]
+    lw, lv = 0.0, 2.0 * L
+    p = np.concatenate([[qw * lw - qv @ lv], qw * lv + lw * qv + np.cross(qv, lv)])[None, :]
+    assert np.allclose(io.body_angular_momentum(q, p), L, atol=1e-15)
+    wb = L / inertia[0]
+    Rz = np.array([[np.cos(phi), -np.sin(phi), 0.0], [np.sin(phi), np.cos(phi), 0.0], [0.0, 0.0, 1.0]])
+    assert np.allclose(io.angular_velocity(q, p, inertia)[0], Rz @ wb, atol=1e-15)
+    v = np.array([[0.1, 0.2, -0.3]])
+    ke = io.kinetic_energy(np.array([3.0]), v, q, p, inertia)
+    assert ke == pytest.approx(0.5 * 3.0 * 0.14 + 0.5 * np.sum(L * L / inertia[0]), rel=1e-14)
+
+
+def test_conf_trajectory_arguments():
+    from oxdna import sim
+    with pytest.raises(ValueError):
+        sim.ConfTrajectory("x.conf", 0)
+    with pytest.raises(ValueError):
+        sim.ConfTrajectory("x.conf", 10, mode="r")
+    w = sim.ConfTrajectory("x.conf", 10)
+    assert w.period == 10 and w.frames == 0
