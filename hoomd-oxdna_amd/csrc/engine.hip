@@ -399,6 +399,14 @@ template<int P, bool WANT_E, bool WANT_V, bool COUNT, bool FUSED = false, bool F
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((FUSED || (WANT_E && WANT_V)) ? 1 : 3)))
 void k_forces(ForceArgs a) {
     constexpr uint32_t TPP = 256 / P;
+    // A grid of at most two workgroups per compute unit runs in two batches: blockIdx
+    // below ncu first (one per compute unit), the rest on compute units that already hold
+    // one, where the instruction arbiter favours the older waves. The second batch's waves
+    // take a higher priority until their first round ends, so the two batches finish
+    // together instead of the second ~19 % behind. (FAR2 instantiations serve the grids
+    // this small; the others are left as they are.)
+    const bool late = FAR2 && gridDim.x <= 2u * a.ncu && blockIdx.x >= a.ncu;
+    if (late) __builtin_amdgcn_s_setprio(1);
     // a run step consumes no energy: its records hold force and torque only, and what
     // only the energies need (the FENE logarithm, ...) is not computed
     constexpr int NBASE = FUSED ? 6 : kRecBase;
@@ -706,6 +714,7 @@ void k_forces(ForceArgs a) {
         for (int q = 0; q < NREC; ++q) s_rec[q][t] = rec[q];
         if (FUSED && r0 == 0 && t / 64u == w_pre && t % 64u < P) pre_particle(t % 64u);
         __syncthreads();
+        if (r0 == 0 && late) __builtin_amdgcn_s_setprio(0);
         // this particle's records of the round: near entries, bond slots, far entries
         if (active) {
             const uint32_t r1 = r0 + 256;
@@ -857,6 +866,7 @@ constexpr uint32_t kMaxNeigh = 768;  // 8 particles x 2 rows x 768 x 4 B = 48 Ki
 
 struct oxdna_ctx {
     int device = 0;
+    uint32_t ncu = 0;  // compute units of the device
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string err;
@@ -1284,6 +1294,7 @@ ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild, u
     a.nlist = c->nlist.p;
     a.head = c->head.p;
     a.far_pairs = fuse_second_half(c) ? 1 : 0;
+    a.ncu = c->ncu;
     a.bpart = c->bpart.p;
     a.binfo = c->binfo.p;
     a.P = c->dparams.p;
@@ -1812,6 +1823,9 @@ int oxdna_ctx_create(int device, void* stream, oxdna_ctx** out) {
         HIP_OK(hipGetDeviceCount(&count));
         if (device < 0 || device >= count) throw Fail("oxdna_ctx_create: no such HIP device");
         HIP_OK(hipSetDevice(device));
+        int ncu = 0;
+        HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+        c->ncu = (uint32_t)ncu;
         if (stream) c->stream = (hipStream_t)stream;
         else {
             HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));

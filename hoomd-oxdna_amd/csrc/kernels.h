@@ -252,6 +252,10 @@ struct StepArgs {
 
 struct ForceArgs {
     uint32_t N;
+    // compute units of the device: a grid of at most two workgroups per compute unit runs
+    // in two dispatch batches, and workgroups ncu and up start in the second (in the
+    // padding after N: the block stays 13 64-byte lines)
+    uint32_t ncu;
     const double4* pos;    // x, y, z, type (low 32 bits of .w)
     const float4* fr0;     // a1, charge
     const float4* fr1;     // a2, type (int bits)
