@@ -231,6 +231,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--lanes", type=int, default=0,
+                    help="force-kernel lanes per particle (4, 8, 16, 32) for the plain run; 0: autotune")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched replica workloads")
     ap.add_argument("--batched-steps", type=int, default=2000)
     ap.add_argument("--plain-sample", type=int, default=32,
@@ -264,7 +266,7 @@ def main():
 
     system = systems.duplex_gas(200, 20, box=200.0, seed=1)
     forces, _ = dna2.forces(fused=True)
-    s = sim.Simulation(device=sim.GPU(0), seed=1, use_graph=not args.no_graph)
+    s = sim.Simulation(device=sim.GPU(0), seed=1, use_graph=not args.no_graph, lanes_per_particle=args.lanes)
     s.create_state_from_snapshot(system.snapshot())
     s.operations.integrator = sim.Integrator(
         dt=0.003, forces=forces, methods=[sim.Langevin(kT=dna1.kT_from_temperature(296.15))])
