@@ -387,6 +387,39 @@ __device__ __forceinline__ uint32_t owner_of(const uint32_t* off, uint32_t x) {
     return lo;
 }
 
+// Reads one dword of each of the 14 64-byte lines at the start of the kernel's argument
+// block (the 832-byte ForceArgs and the first line of the hidden arguments after it,
+// which the kernel also reads), all issued before one wait in the same asm block: the
+// compiler's own reads of the arguments, one dependent round trip each on the way to
+// the first barrier, then hit the scalar cache. Returns their XOR for a sink.
+__device__ __forceinline__ uint32_t prefetch_args() {
+    static_assert(sizeof(ForceArgs) <= 14 * 64, "the prefetch covers 14 lines");
+    const uint64_t base = (uint64_t)__builtin_amdgcn_kernarg_segment_ptr();
+    uint32_t r[14];
+    asm volatile(
+        "s_load_dword %0, %14, 0x0\n\t"
+        "s_load_dword %1, %14, 0x40\n\t"
+        "s_load_dword %2, %14, 0x80\n\t"
+        "s_load_dword %3, %14, 0xc0\n\t"
+        "s_load_dword %4, %14, 0x100\n\t"
+        "s_load_dword %5, %14, 0x140\n\t"
+        "s_load_dword %6, %14, 0x180\n\t"
+        "s_load_dword %7, %14, 0x1c0\n\t"
+        "s_load_dword %8, %14, 0x200\n\t"
+        "s_load_dword %9, %14, 0x240\n\t"
+        "s_load_dword %10, %14, 0x280\n\t"
+        "s_load_dword %11, %14, 0x2c0\n\t"
+        "s_load_dword %12, %14, 0x300\n\t"
+        "s_load_dword %13, %14, 0x340\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&s"(r[0]), "=&s"(r[1]), "=&s"(r[2]), "=&s"(r[3]), "=&s"(r[4]), "=&s"(r[5]), "=&s"(r[6]), "=&s"(r[7]), "=&s"(r[8]), "=&s"(r[9]), "=&s"(r[10]), "=&s"(r[11]), "=&s"(r[12]), "=&s"(r[13])
+        : "s"(base));
+    uint32_t x = 0;
+#pragma unroll
+    for (int k = 0; k < 14; ++k) x ^= r[k];
+    return x;
+}
+
 // FUSED: a run step whose epilogue also takes the particle through the Langevin second
 // half of step k and the first half of step k + 1 (a.integrate == 2); the force-free part
 // of that runs during the first round, and the force and torque, consumed in the
@@ -407,6 +440,8 @@ void k_forces(ForceArgs a) {
     // this small; the others are left as they are.)
     const bool late = FAR2 && gridDim.x <= 2u * a.ncu && blockIdx.x >= a.ncu;
     if (late) __builtin_amdgcn_s_setprio(1);
+    uint32_t arg_lines = 0;
+    if constexpr (FAR2) arg_lines = prefetch_args();
     // a run step consumes no energy: its records hold force and torque only, and what
     // only the energies need (the FENE logarithm, ...) is not computed
     constexpr int NBASE = FUSED ? 6 : kRecBase;
@@ -566,6 +601,7 @@ void k_forces(ForceArgs a) {
         }
     }
     __syncthreads();
+    if constexpr (FAR2) asm volatile("" ::"s"(arg_lines));
     if (t < 3) {  // exclusive prefix sums over the P particles, one thread per count
         uint32_t v[P];
 #pragma unroll
