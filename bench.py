@@ -34,6 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "hoomd-oxdna_amd"))
 
 FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32 (published)
+HBM_PEAK_BPS = 8.0e12   # MI355X HBM3E (published)
 TERM_NAMES = ["excvol", "hbond", "cross", "coax", "debye", "hbondcross", "bonded"]
 
 
@@ -58,7 +59,7 @@ def flops_per_launch(counts, census, evals_per_pair=1):
 def traffic_from_profile(kernel):
     """HBM bytes per force launch from the committed rocprofv3 counter summary, when it
     was taken on the same kernel instantiation (else None)."""
-    path = os.path.join(ROOT, "profiles", "k_forces_oxdna2_8k_traffic.json")
+    path = os.path.join(ROOT, "profiles", "k_forces_fused_oxdna2_8k_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -287,21 +288,29 @@ def main():
     e = s.term_energies()
     assert all(np.isfinite(v) for v in e.values()), e
 
-    # dominant kernel: the fused force kernel, timed with HIP events on the engine stream
+    # dominant kernel: the run step's own launch (k_forces FUSED: every term + the
+    # Langevin integrator in its epilogue), timed with HIP events on the engine stream
+    # around graph replays of run steps; beside it the forces-only instantiation
     import ctypes as C
     lib, ctx = s._lib, s._ctx
-    avg_ms = C.c_double()
-    assert lib.oxdna_time_forces(ctx, s._mask, 200, C.byref(avg_ms)) == 0
     counts = (C.c_uint64 * 14)()
     assert lib.oxdna_count_evaluations(ctx, s._mask, counts) == 0
     counts = list(counts)
     with open(os.path.join(ROOT, "hoomd-oxdna_amd", "oxdna", "data", "census_flops.json")) as fh:
         census = json.load(fh)["oxdna2_fused"]
     flops = flops_per_launch(counts, census, evals_per_pair=2)
-    achieved = flops / (avg_ms.value * 1e-3) / 1e12
+    step_ms = C.c_double()
+    assert lib.oxdna_time_steps(ctx, s._mask, 1 + 4 * 96, C.byref(step_ms)) == 0
+    fo_ms = C.c_double()
+    assert lib.oxdna_time_forces(ctx, s._mask, 200, C.byref(fo_ms)) == 0
+    achieved = flops / (step_ms.value * 1e-3) / 1e12
+    achieved_fo = flops / (fo_ms.value * 1e-3) / 1e12
     pair_evals = sum(counts[2 * t] for t in range(6)) // 2
     tpp = C.c_int()
     assert lib.oxdna_get_tuning(ctx, C.byref(tpp), None) == 0
+    P = 256 // tpp.value
+    kname = f"k_forces<{P},0,0,0,FUSED,FAR2>"
+    traffic = traffic_from_profile(kname)
     stats = s.stats()
 
     line = {
@@ -322,9 +331,14 @@ def main():
                    "n_nucleotides": system.n, "graph": not args.no_graph},
         "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 6),
-                     "traffic": traffic_from_profile(f"k_forces<{256 // tpp.value},"),
-                     "kernel": f"k_forces<{256 // tpp.value}>",
-                     "kernel_ms": round(avg_ms.value, 5), "flops_per_launch": flops},
+                     "traffic": traffic,
+                     "hbm_frac": (round(traffic / (step_ms.value * 1e-3) / HBM_PEAK_BPS, 5)
+                                  if traffic else None),
+                     "kernel": kname + " (the run step's launch: forces + Langevin integrator)",
+                     "kernel_ms": round(step_ms.value, 5), "flops_per_launch": flops},
+        "roofline_forces_only": {"kernel": f"k_forces<{P},0,0,0,0,FAR2> (forces only, oxdna_time_forces)",
+                                 "kernel_ms": round(fo_ms.value, 5), "achieved": round(achieved_fo, 4),
+                                 "frac": round(achieved_fo / FP32_PEAK_TFLOPS, 6)},
         "pair_evaluations_per_s": round(pair_evals * steps_per_s, 1),
         "nlist_builds": stats["nlist_builds"],
         "energy_per_nt": {k: round(v / system.n, 6) for k, v in e.items()},

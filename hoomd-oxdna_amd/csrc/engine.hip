@@ -387,14 +387,17 @@ __device__ __forceinline__ uint32_t owner_of(const uint32_t* off, uint32_t x) {
     return lo;
 }
 
-// Reads one dword of each of the 14 64-byte lines at the start of the kernel's argument
-// block (the 832-byte ForceArgs and the first line of the hidden arguments after it,
-// which the kernel also reads), all issued before one wait in the same asm block: the
-// compiler's own reads of the arguments, one dependent round trip each on the way to
-// the first barrier, then hit the scalar cache. Returns their XOR for a sink.
+// Reads one dword of each of the 13 64-byte lines of the kernel's explicit arguments
+// (the 832-byte ForceArgs) and of the first line of its hidden arguments (grid size,
+// which the kernel reads: the address comes from the implicit-argument pointer, so the
+// line exists whenever this is called), all issued before one wait in the same asm
+// block: the compiler's own reads of the arguments, one dependent round trip each on
+// the way to the first barrier, then hit the scalar cache. Returns their XOR for a sink.
 __device__ __forceinline__ uint32_t prefetch_args() {
-    static_assert(sizeof(ForceArgs) <= 14 * 64, "the prefetch covers 14 lines");
+    static_assert(sizeof(ForceArgs) > 12 * 64 && sizeof(ForceArgs) <= 13 * 64,
+                  "the prefetch names the 13 lines of ForceArgs: update it with the struct");
     const uint64_t base = (uint64_t)__builtin_amdgcn_kernarg_segment_ptr();
+    const uint64_t hidden = (uint64_t)__builtin_amdgcn_implicitarg_ptr();
     uint32_t r[14];
     asm volatile(
         "s_load_dword %0, %14, 0x0\n\t"
@@ -410,10 +413,10 @@ __device__ __forceinline__ uint32_t prefetch_args() {
         "s_load_dword %10, %14, 0x280\n\t"
         "s_load_dword %11, %14, 0x2c0\n\t"
         "s_load_dword %12, %14, 0x300\n\t"
-        "s_load_dword %13, %14, 0x340\n\t"
+        "s_load_dword %13, %15, 0x0\n\t"
         "s_waitcnt lgkmcnt(0)"
         : "=&s"(r[0]), "=&s"(r[1]), "=&s"(r[2]), "=&s"(r[3]), "=&s"(r[4]), "=&s"(r[5]), "=&s"(r[6]), "=&s"(r[7]), "=&s"(r[8]), "=&s"(r[9]), "=&s"(r[10]), "=&s"(r[11]), "=&s"(r[12]), "=&s"(r[13])
-        : "s"(base));
+        : "s"(base), "s"(hidden));
     uint32_t x = 0;
 #pragma unroll
     for (int k = 0; k < 14; ++k) x ^= r[k];
@@ -495,6 +498,7 @@ void k_forces(ForceArgs a) {
     // epilogue
     __shared__ double4 s_st[FUSED ? 4 : 1][FUSED ? P : 1];  // v, p, q, position of the last rebuild
     __shared__ uint32_t s_bin[2][FUSED ? P : 1];            // predicted cell and slot
+    __shared__ double s_fene[WANT_E ? 2 * P : 1];           // FP64 FENE energy of each bond slot
     auto pre_particle = [&](uint32_t ps) {
         const uint32_t ip = xcd_block() * P + ps;
         if (ip >= a.N) return;
@@ -623,6 +627,7 @@ void k_forces(ForceArgs a) {
 
     // ---- (2) entries, 256 per round; (3) each particle's lanes sum its records
     float et[8];
+    double et_fene = 0.0;  // the FENE energy stays FP64 into the block reduction
     uint32_t cnt[14];
 #pragma unroll
     for (int k = 0; k < 8; ++k) et[k] = 0.f;
@@ -712,6 +717,7 @@ void k_forces(ForceArgs a) {
             const uint32_t ib = xcd_block() * P + s;
             const bool pre = bpre && r0 == rb0;
             const int other = pre ? b_other : (ib < a.N ? a.bpart[2 * ib + ((e - B0) & 1u)] : -1);
+            if (WANT_E && other < 0) s_fene[e - B0] = 0.0;
             if (other >= 0) {
                 const int info = pre ? b_info : a.binfo[2 * ib + ((e - B0) & 1u)];
                 const int role = info & 1;
@@ -735,7 +741,8 @@ void k_forces(ForceArgs a) {
                 // per-term sums count every directed evaluation in full and are halved at
                 // the end (each pair and each bond is evaluated from both of its ends)
                 const float ef = (float)r.e_fene;
-                et[S_FENE] += ef;
+                et_fene += r.e_fene;
+                if (WANT_E) s_fene[e - B0] = r.e_fene;
                 et[S_BEXC] += r.e_bexc;
                 et[S_STACK] += r.e_stack;
                 if constexpr (NBASE > 6) rec[6] = 0.5f * (ef + r.e_bexc + r.e_stack);
@@ -776,8 +783,10 @@ void k_forces(ForceArgs a) {
     // reduce the TPP lanes of a particle (adjacent lanes of one wavefront)
 #pragma unroll
     for (int q = 0; q < NREC; ++q) acc[q] = lane_sum<TPP>(acc[q]);
-    if (WANT_E && a.split && active && lane == 0)
-        a.split[i] = make_double4(0.5 * acc[RS + 0], 0.5 * acc[RS + 1], 0.5 * acc[RS + 2], 0.0);
+    if (WANT_E && a.split && active && lane == 0) {
+        const double fene = (a.mask & (1u << T_BONDED)) ? s_fene[2 * slot] + s_fene[2 * slot + 1] : 0.0;
+        a.split[i] = make_double4(0.5 * fene, 0.5 * acc[RS + 1], 0.5 * acc[RS + 2], 0.0);
+    }
     if (active && lane == 0) {
         float ehalf = 0.f;
         if constexpr (NBASE > 6) ehalf = acc[6];
@@ -824,7 +833,7 @@ void k_forces(ForceArgs a) {
         const int wid = t / 64, wl = t % 64;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            double v = (double)et[q];
+            double v = q == S_FENE ? et_fene : (double)et[q];
 #pragma unroll
             for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
             if (wl == 0) red[wid][q] = v;
@@ -852,6 +861,62 @@ __global__ void k_reduce_energy(uint32_t nblocks, const double* __restrict__ par
         }
         if (threadIdx.x == 0) out[slot] = (accumulate ? out[slot] : 0.0) + s[0];
         __syncthreads();
+    }
+}
+
+// The per-particle state a run chunk starts from: what a chunk whose cells or rows
+// overflowed is run again from.
+struct RunState {
+    double4 *pos, *vel, *quat, *angmom, *pos_last, *bd, *bd_tq;
+    float4 *force, *torque;
+};
+
+// Copies `from` into `to` (N particles). Start of a run chunk (f != NULL): also zeroes
+// the ncount cell counts and numbers the chunk's steps from step0, with no rebuild
+// request pending and the overflow record cleared.
+__global__ void k_chunk_begin(uint32_t N, RunState from, RunState to, uint32_t* counts, size_t ncount, DevFlags* f,
+                              unsigned long long step0) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = g; k < N; k += stride) {
+        to.pos[k] = from.pos[k]; to.vel[k] = from.vel[k]; to.quat[k] = from.quat[k];
+        to.angmom[k] = from.angmom[k]; to.pos_last[k] = from.pos_last[k];
+        to.bd[k] = from.bd[k]; to.bd_tq[k] = from.bd_tq[k];
+        to.force[k] = from.force[k]; to.torque[k] = from.torque[k];
+    }
+    for (size_t k = g; k < ncount; k += stride) counts[k] = 0u;
+    if (f && g == 0) {
+        f->stepc[0] = step0;
+        f->rebuild[0] = f->rebuild[1] = f->rebuild[2] = 0u;
+        f->overflow = 0u; f->max_cell = 0u; f->max_neigh = 0u;
+        f->step = step0;
+    }
+}
+
+// What the host reads after a run chunk (pinned host memory, written by k_chunk_end).
+struct RunStatus {
+    uint32_t overflow, max_cell, max_neigh, done;
+    unsigned long long builds;
+};
+
+// End of a run chunk: its state is in buffer `odd` of the position and frame pairs;
+// bring it to buffer 0 (where every other entry point reads it), record the step count
+// and report the overflow record to the host.
+__global__ void k_chunk_end(uint32_t N, int odd, double4* pos, float4* fr0, float4* fr1, float4* fr2, DevFlags* f,
+                            unsigned long long step_end, RunStatus* out) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+    if (odd) {
+        for (size_t k = g; k < N; k += stride) {
+            pos[k] = pos[N + k];
+            fr0[k] = fr0[N + k]; fr1[k] = fr1[N + k]; fr2[k] = fr2[N + k];
+        }
+    }
+    if (g == 0) {
+        f->step = step_end;
+        out->overflow = f->overflow;
+        out->max_cell = f->max_cell;
+        out->max_neigh = f->max_neigh;
+        out->builds = f->builds;
+        out->done = 1u;
     }
 }
 
@@ -946,9 +1011,20 @@ struct oxdna_ctx {
     uint32_t tuned_n = 0;
     int fixed_tpp = 0;                   // nonzero: lanes per particle pinned by the caller
     uint32_t forces_mask = 0xffffffffu;  // mask of the forces currently in `force`
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr;      // kGraphSteps inner steps
+    hipGraphExec_t graph6 = nullptr;     // 6 inner steps (the tail of a chunk)
     uint32_t graph_mask = 0;
     uint64_t host_step = 0;
+    // the state a run chunk starts from (restored when the chunk must run again)
+    DBuf<double4> sv_pos, sv_vel, sv_quat, sv_angmom, sv_pos_last, sv_bd, sv_bd_tq;
+    DBuf<float4> sv_force, sv_torque;
+    RunStatus* status_h = nullptr;       // pinned host memory written by k_chunk_end
+    RunStatus* status_d = nullptr;       // its device address
+    // oxdna_time_steps: HIP events around a chunk's graph replays
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double timed_ms = 0.0;
+    uint64_t timed_steps = 0;
 
     // batched replicas (n_rep > 0): particles [rep_off[r], rep_off[r+1]) form replica r
     uint32_t n_rep = 0;
@@ -964,6 +1040,12 @@ struct oxdna_ctx {
 
     ~oxdna_ctx() {
         if (graph) (void)hipGraphExecDestroy(graph);
+        if (graph6) (void)hipGraphExecDestroy(graph6);
+        sv_pos.release(); sv_vel.release(); sv_quat.release(); sv_angmom.release(); sv_pos_last.release();
+        sv_bd.release(); sv_bd_tq.release(); sv_force.release(); sv_torque.release();
+        if (status_h) (void)hipHostFree(status_h);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
         pos.release(); vel.release(); quat.release(); angmom.release(); pos_last.release();
         bd.release(); bd_tq.release();
         charge.release(); fr0.release(); fr1.release(); fr2.release(); force.release(); torque.release();
@@ -977,7 +1059,9 @@ struct oxdna_ctx {
 
     void drop_graph() {
         if (graph) (void)hipGraphExecDestroy(graph);
+        if (graph6) (void)hipGraphExecDestroy(graph6);
         graph = nullptr;
+        graph6 = nullptr;
     }
 };
 
@@ -1517,17 +1601,27 @@ void enqueue_last(oxdna_ctx* c, uint32_t mask, uint64_t k) {
 }
 
 // Inner steps per graph: a multiple of 6, so that its buffer indices (step mod 2 and
-// mod 3) are the same at every replay.
+// mod 3) are the same at every replay. A chunk replays the long graph, then the short
+// one, and launches the last (< 6) inner steps directly.
 constexpr uint32_t kGraphSteps = 96;
+constexpr uint32_t kGraphStepsShort = 6;
+
+hipGraphExec_t capture_inner(oxdna_ctx* c, uint32_t mask, uint32_t len) {
+    hipGraph_t g;
+    hipGraphExec_t x = nullptr;
+    HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_inner(c, mask, 0, len);
+    HIP_OK(hipStreamEndCapture(c->stream, &g));
+    const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_OK(e);
+    return x;
+}
 
 void build_graph(oxdna_ctx* c, uint32_t mask) {
     c->drop_graph();
-    hipGraph_t g;
-    HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_inner(c, mask, 0, kGraphSteps);
-    HIP_OK(hipStreamEndCapture(c->stream, &g));
-    HIP_OK(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
-    HIP_OK(hipGraphDestroy(g));
+    c->graph = capture_inner(c, mask, kGraphSteps);
+    c->graph6 = capture_inner(c, mask, kGraphStepsShort);
     c->graph_mask = mask;
 }
 
@@ -1573,94 +1667,87 @@ void autotune(oxdna_ctx* c, uint32_t m) {
     // the timing launches overwrote the forces with identical values; nothing to redo
 }
 
-struct Saved {
-    DBuf<double4> pos, vel, quat, angmom, pos_last, bd, bd_tq;
-    DBuf<float4> force, torque;
-    ~Saved() { pos.release(); vel.release(); quat.release(); angmom.release(); pos_last.release();
-               bd.release(); bd_tq.release();
-               force.release(); torque.release(); }
-};
-
-void save_state(oxdna_ctx* c, Saved& s) {
-    auto cp = [&](auto& dst, auto& src) {
-        dst.alloc(c->N);
-        HIP_OK(hipMemcpyAsync(dst.p, src.p, sizeof(*src.p) * c->N, hipMemcpyDeviceToDevice, c->stream));
-    };
-    cp(s.pos, c->pos); cp(s.vel, c->vel); cp(s.quat, c->quat); cp(s.angmom, c->angmom);
-    cp(s.pos_last, c->pos_last); cp(s.force, c->force); cp(s.torque, c->torque);
-    cp(s.bd, c->bd); cp(s.bd_tq, c->bd_tq);
-}
-void restore_state(oxdna_ctx* c, Saved& s) {
-    auto cp = [&](auto& dst, auto& src) {
-        HIP_OK(hipMemcpyAsync(dst.p, src.p, sizeof(*src.p) * c->N, hipMemcpyDeviceToDevice, c->stream));
-    };
-    cp(c->pos, s.pos); cp(c->vel, s.vel); cp(c->quat, s.quat); cp(c->angmom, s.angmom);
-    cp(c->pos_last, s.pos_last); cp(c->force, s.force); cp(c->torque, s.torque);
-    cp(c->bd, s.bd); cp(c->bd_tq, s.bd_tq);
+RunState live_state(oxdna_ctx* c) {
+    return RunState{c->pos.p, c->vel.p, c->quat.p, c->angmom.p, c->pos_last.p,
+                    c->bd.p, c->bd_tq.p, c->force.p, c->torque.p};
 }
 
-// Runs `nsteps` Langevin steps in chunks; a chunk whose cells or rows overflowed is
-// run again from its saved start with larger capacities.
+RunState saved_state(oxdna_ctx* c) {
+    c->sv_pos.alloc(c->N); c->sv_vel.alloc(c->N); c->sv_quat.alloc(c->N); c->sv_angmom.alloc(c->N);
+    c->sv_pos_last.alloc(c->N); c->sv_bd.alloc(c->N); c->sv_bd_tq.alloc(c->N);
+    c->sv_force.alloc(c->N); c->sv_torque.alloc(c->N);
+    return RunState{c->sv_pos.p, c->sv_vel.p, c->sv_quat.p, c->sv_angmom.p, c->sv_pos_last.p,
+                    c->sv_bd.p, c->sv_bd_tq.p, c->sv_force.p, c->sv_torque.p};
+}
+
+uint32_t grid_stride_blocks(size_t n) { return (uint32_t)std::max<size_t>(1, std::min<size_t>(blocks_for(n), 2048)); }
+
+// Runs `nsteps` Langevin steps in chunks of up to 42 graph lengths. A chunk costs the
+// host two launches of its own (k_chunk_begin: state saved, cells emptied, steps
+// numbered; k_chunk_end: state to buffer 0, status into pinned host memory) and one
+// stream synchronisation; a chunk whose cells or rows overflowed is run again from
+// the saved start with larger capacities.
 void run(oxdna_ctx* c, uint32_t nsteps, uint32_t mask, int use_graph) {
     if (!c->have_langevin) throw Fail("no integrator: call oxdna_set_langevin first");
     ensure_ready(c);
     uint32_t m = effective_mask(c, mask);
     if (c->forces_mask != m || !c->nlist_valid) compute(c, m, 1);
     autotune(c, m);
+    // the graphs are captured and instantiated once, at the first run of a setup (its
+    // cost, ~1 ms, goes there and not into the first chunk long enough to replay them)
+    if (use_graph && (!c->graph || c->graph_mask != m)) build_graph(c, m);
     const uint32_t chunk_max = 42 * kGraphSteps + 1;
+    const bool pairs = (m & 0x3fu) != 0;
     uint32_t done = 0;
-    Saved saved;
     while (done < nsteps) {
         const uint32_t chunk = std::min(chunk_max, nsteps - done);
-        save_state(c, saved);
-        DevFlags f0;
-        read_flags(c, f0);
-        const unsigned long long step0 = f0.step;
+        // the chunk's steps are numbered 1 ... chunk from its starting state (step 0):
+        // random-stream counter, empty cells, no rebuild requests
+        const unsigned long long step0 = c->host_step + done;
         for (int attempt = 0;; ++attempt) {
-            // the chunk's steps are numbered 1 ... chunk from its starting state (step 0):
-            // random-stream counter, empty cells, no rebuild requests
-            f0.stepc[0] = step0;
-            f0.rebuild[0] = f0.rebuild[1] = f0.rebuild[2] = 0;
-            HIP_OK(hipMemcpyAsync(c->flags.p, &f0, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
-            zero_counts(c);
-            const bool pairs = (m & 0x3fu) != 0;
+            const size_t ncount = pairs ? 3 * ncell_total(c) : 0;
+            hipLaunchKernelGGL(k_chunk_begin, dim3(grid_stride_blocks(std::max<size_t>(c->N, ncount))), dim3(kBlock),
+                               0, c->stream, c->N, live_state(c), saved_state(c), c->cell_count.p, ncount, c->flags.p,
+                               step0);
             hipLaunchKernelGGL(k_step1, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, step_args(c, pairs, 1));
             uint32_t k = 0;  // inner steps enqueued
-            if (use_graph && chunk - 1 >= kGraphSteps) {
+            const bool timed = c->timing && use_graph && chunk - 1 >= kGraphStepsShort;
+            if (use_graph && chunk - 1 >= kGraphStepsShort) {
                 if (!c->graph || c->graph_mask != m) build_graph(c, m);
+                if (timed) HIP_OK(hipEventRecord(c->ev0, c->stream));
                 for (; k + kGraphSteps <= chunk - 1; k += kGraphSteps) HIP_OK(hipGraphLaunch(c->graph, c->stream));
+                for (; k + kGraphStepsShort <= chunk - 1; k += kGraphStepsShort)
+                    HIP_OK(hipGraphLaunch(c->graph6, c->stream));
+                if (timed) HIP_OK(hipEventRecord(c->ev1, c->stream));
             }
             // the rest without the graph: the same launches, so both paths are bit-identical
             enqueue_inner(c, m, k, chunk - 1 - k);
             enqueue_last(c, m, chunk);
-            DevFlags f;
-            read_flags(c, f);
-            if (f.overflow == 0) {
-                // the chunk's state is in buffer chunk mod 2: bring it to buffer 0
-                if (chunk & 1u) {
-                    const size_t n = c->N;
-                    HIP_OK(hipMemcpyAsync(c->pos.p, c->pos.p + n, sizeof(double4) * n, hipMemcpyDeviceToDevice,
-                                          c->stream));
-                    for (DBuf<float4>* b : {&c->fr0, &c->fr1, &c->fr2})
-                        HIP_OK(hipMemcpyAsync(b->p, b->p + n, sizeof(float4) * n, hipMemcpyDeviceToDevice,
-                                              c->stream));
-                }
-                f.step = step0 + chunk;
-                HIP_OK(hipMemcpyAsync(c->flags.p, &f, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
-                HIP_OK(hipStreamSynchronize(c->stream));
-                break;
+            c->status_h->done = 0u;
+            hipLaunchKernelGGL(k_chunk_end, dim3(grid_stride_blocks(c->N)), dim3(kBlock), 0, c->stream, c->N,
+                               (int)(chunk & 1u), c->pos.p, c->fr0.p, c->fr1.p, c->fr2.p, c->flags.p,
+                               step0 + chunk, c->status_d);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipStreamSynchronize(c->stream));
+            const volatile RunStatus* st = c->status_h;
+            if (st->done != 1u) throw Fail("internal: run chunk status not written");
+            if (timed) {
+                float ms = 0.f;
+                HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+                c->timed_ms += ms;
+                c->timed_steps += k;
             }
+            if (st->overflow == 0u) break;
             if (attempt >= 4) throw Fail("neighbour list kept overflowing during the run");
             // grow, restore the chunk's starting state and run it again
-            restore_state(c, saved);
-            c->max_neigh = std::max(c->max_neigh, (uint32_t)std::ceil(f.max_neigh * 1.5) + 8);
-            c->cell_cap = std::max(c->cell_cap, (uint32_t)std::ceil(f.max_cell * 1.5) + 4);
-            f.overflow = 0; f.step = step0;
-            HIP_OK(hipMemcpyAsync(c->flags.p, &f, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
+            const uint32_t max_neigh = st->max_neigh, max_cell = st->max_cell;
+            hipLaunchKernelGGL(k_chunk_begin, dim3(grid_stride_blocks(c->N)), dim3(kBlock), 0, c->stream, c->N,
+                               saved_state(c), live_state(c), (uint32_t*)nullptr, (size_t)0, (DevFlags*)nullptr, 0ull);
+            c->max_neigh = std::max(c->max_neigh, (uint32_t)std::ceil(max_neigh * 1.5) + 8);
+            c->cell_cap = std::max(c->cell_cap, (uint32_t)std::ceil(max_cell * 1.5) + 4);
             c->drop_graph();
             setup_cells(c);
             compute(c, m, 1);
-            read_flags(c, f0);
         }
         done += chunk;
     }
@@ -1870,6 +1957,11 @@ int oxdna_ctx_create(int device, void* stream, oxdna_ctx** out) {
         c->flags.alloc(1);
         DevFlags f{};
         HIP_OK(hipMemcpy(c->flags.p, &f, sizeof(f), hipMemcpyHostToDevice));
+        // the run status: written by the GPU at the end of each chunk, read by the host
+        // after the stream synchronisation (coherent: no cache flush between them)
+        HIP_OK(hipHostMalloc((void**)&c->status_h, sizeof(RunStatus), hipHostMallocMapped | hipHostMallocCoherent));
+        std::memset(c->status_h, 0, sizeof(RunStatus));
+        HIP_OK(hipHostGetDevicePointer((void**)&c->status_d, c->status_h, 0));
         c->energy.alloc(8);
         HIP_OK(hipMemset(c->energy.p, 0, 8 * sizeof(double)));
     });
@@ -2324,6 +2416,30 @@ int oxdna_time_forces(oxdna_ctx* c, unsigned int mask, unsigned int nrep, double
     });
 }
 
+int oxdna_time_steps(oxdna_ctx* c, unsigned int mask, unsigned int nsteps, double* avg_ms) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (!avg_ms) throw Fail("oxdna_time_steps: avg_ms is NULL");
+        if (c->n_rep) throw Fail("oxdna_time_steps: one-system contexts only (replicas: oxdna_time_forces)");
+        if (nsteps < 1 + kGraphStepsShort)
+            throw Fail("oxdna_time_steps: nsteps must be at least " + std::to_string(1 + kGraphStepsShort));
+        if (!c->ev0) HIP_OK(hipEventCreate(&c->ev0));
+        if (!c->ev1) HIP_OK(hipEventCreate(&c->ev1));
+        c->timed_ms = 0.0;
+        c->timed_steps = 0;
+        c->timing = true;
+        try {
+            run(c, nsteps, mask, 1);
+        } catch (...) {
+            c->timing = false;
+            throw;
+        }
+        c->timing = false;
+        if (c->timed_steps == 0) throw Fail("oxdna_time_steps: no graph-launched step was timed");
+        *avg_ms = c->timed_ms / (double)c->timed_steps;
+    });
+}
+
 int oxdna_count_evaluations(oxdna_ctx* c, unsigned int mask, uint64_t out[14]) {
     if (!c) return fail(nullptr, "NULL context");
     return guard(c, [&] {
@@ -2439,6 +2555,65 @@ void driver_frames(DriverScratch& s, uint32_t N, const double* d_pos, const doub
                        (const double4*)d_pos, s.fr0.p, s.fr1.p, s.fr2.p);
 }
 
+// Lanes per particle of the stateless drivers' force launches.
+constexpr int kDriverTpp = 8;
+
+// Checks the arguments of a stateless pair-driver call and builds its parameter block.
+DParams pair_driver_params(const char* who, int term, unsigned int N, float* d_force, float* d_torque,
+                           const double* d_pos, const double* d_orientation, const double* d_charge,
+                           const oxdna_box* box, const unsigned int* d_n_neigh, const unsigned int* d_nlist,
+                           const size_t* d_head_list, int ntypes, const void* params, const double* rcutsq) {
+    const std::string w(who);
+    if (term < 0 || term >= OXDNA_TERM_BONDED) throw Fail(w + ": not a nonbonded term id");
+    if (ntypes < 1 || ntypes > 4) throw Fail(w + ": ntypes must be 1..4");
+    if (!params || !rcutsq || !box) throw Fail(w + ": NULL parameter table, cut-offs or box");
+    DParams P{};
+    load_pair_params(P, term, ntypes, params);
+    for (int k = 0; k < ntypes * ntypes; ++k) {
+        if (!(rcutsq[k] >= 0.0)) throw Fail(w + ": negative or NaN squared cut-off");
+        P.rcutsq[term][k] = (float)rcutsq[k];
+    }
+    P.ntypes = ntypes;
+    P.mask = 1u << term;
+    if (N == 0) return P;
+    if (!d_force || !d_torque || !d_pos || !d_orientation || !d_n_neigh || !d_nlist || !d_head_list)
+        throw Fail(w + ": NULL device array");
+    if (term == OXDNA_TERM_DEBYE && !d_charge) throw Fail(w + ": Debye-Hueckel needs charges");
+    if (N > kIndexMask) throw Fail(w + ": N must be below 2^30 (neighbour indices keep 30 bits)");
+    return P;
+}
+
+// Frames, the force launch over the caller's rows and the energy reduction of one
+// stateless pair-driver call, on the given device workspace (dP: the parameter block
+// on the device; fr0..fr2: N float4 each; epart: 8 doubles per force workgroup).
+void pair_driver_launch(int term, float* d_force, float* d_torque, float* d_virial, size_t virial_pitch,
+                        unsigned int N, const double* d_pos, const double* d_orientation, const double* d_charge,
+                        const oxdna_box* box, const unsigned int* d_n_neigh, const unsigned int* d_nlist,
+                        const size_t* d_head_list, const DParams* dP, float4* fr0, float4* fr1, float4* fr2,
+                        double* epart, double* d_term_energy, hipStream_t st) {
+    hipLaunchKernelGGL(k_frames, dim3(blocks_for(N)), dim3(kBlock), 0, st, N, (const double4*)d_orientation, d_charge,
+                       (const double4*)d_pos, fr0, fr1, fr2);
+    const uint32_t blocks = blocks_for((size_t)N * kDriverTpp);
+    ForceArgs a{};
+    a.N = N;
+    a.pos = (const double4*)d_pos;
+    a.fr0 = fr0; a.fr1 = fr1; a.fr2 = fr2;
+    // the caller's rows are only read (no rebuild without a flags block)
+    a.n_neigh = const_cast<uint32_t*>(d_n_neigh);
+    a.nlist = const_cast<uint32_t*>(d_nlist);
+    a.head = d_head_list;
+    a.P = dP;
+    for (int k = 0; k < 3; ++k) { a.L[k] = box->L[k]; a.Linv[k] = 1.0 / box->L[k]; }
+    a.mask = 1u << term;
+    a.force = (float4*)d_force; a.torque = (float4*)d_torque;
+    a.virial = d_virial; a.vpitch = virial_pitch;
+    a.epart = epart;
+    const bool we = d_term_energy != nullptr, wv = d_virial != nullptr;
+    launch_k_any(kDriverTpp, we, wv, dim3(blocks), dim3(kBlock), 0, st, a);
+    if (we) hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(256), 0, st, blocks, epart, d_term_energy, 0);
+    HIP_OK(hipGetLastError());
+}
+
 __global__ void k_bond_table(uint32_t N, const uint32_t* blist, const uint32_t* bpos, uint32_t pitch,
                              const uint32_t* n_bonds, int* bpart, int* binfo, uint32_t* err) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2466,45 +2641,18 @@ int oxdna_pair_forces(int term, float* d_force, float* d_torque, float* d_virial
                       const size_t* d_head_list, int ntypes, const void* params, const double* rcutsq,
                       double* d_term_energy, void* stream) {
     return guard(nullptr, [&] {
-        if (term < 0 || term >= OXDNA_TERM_BONDED) throw Fail("oxdna_pair_forces: not a nonbonded term id");
-        if (ntypes < 1 || ntypes > 4) throw Fail("oxdna_pair_forces: ntypes must be 1..4");
-        if (!params || !rcutsq || !box) throw Fail("oxdna_pair_forces: NULL parameter table, cut-offs or box");
+        const DParams P = pair_driver_params("oxdna_pair_forces", term, N, d_force, d_torque, d_pos, d_orientation,
+                                             d_charge, box, d_n_neigh, d_nlist, d_head_list, ntypes, params, rcutsq);
         if (N == 0) return;
-        if (!d_force || !d_torque || !d_pos || !d_orientation || !d_n_neigh || !d_nlist || !d_head_list)
-            throw Fail("oxdna_pair_forces: NULL device array");
-        if (term == OXDNA_TERM_DEBYE && !d_charge) throw Fail("oxdna_pair_forces: Debye-Hueckel needs charges");
-        if (N > kIndexMask) throw Fail("oxdna_pair_forces: N must be below 2^30 (neighbour indices keep 30 bits)");
         hipStream_t st = (hipStream_t)stream;
         int dev;
         DriverScratch& s = scratch_for(dev);
-        DParams P{};
-        load_pair_params(P, term, ntypes, params);
-        for (int k = 0; k < ntypes * ntypes; ++k) P.rcutsq[term][k] = (float)rcutsq[k];
-        P.ntypes = ntypes;
-        P.mask = 1u << term;
         const DParams* dP = stage_params(s, P, st);
-        driver_frames(s, N, d_pos, d_orientation, d_charge, st);
-        const int tpp = 8;
-        uint32_t blocks = blocks_for((size_t)N * tpp);
-        s.epart.alloc((size_t)blocks * 8);
-        ForceArgs a{};
-        a.N = N;
-        a.pos = (const double4*)d_pos;
-        a.fr0 = s.fr0.p; a.fr1 = s.fr1.p; a.fr2 = s.fr2.p;
-        // the caller's rows are only read (no rebuild without a flags block)
-        a.n_neigh = const_cast<uint32_t*>(d_n_neigh);
-        a.nlist = const_cast<uint32_t*>(d_nlist);
-        a.head = d_head_list;
-        a.P = dP;
-        for (int k = 0; k < 3; ++k) { a.L[k] = box->L[k]; a.Linv[k] = 1.0 / box->L[k]; }
-        a.mask = 1u << term;
-        a.force = (float4*)d_force; a.torque = (float4*)d_torque;
-        a.virial = d_virial; a.vpitch = virial_pitch;
-        a.epart = s.epart.p;
-        bool we = d_term_energy != nullptr, wv = d_virial != nullptr;
-        launch_k_any(tpp, we, wv, dim3(blocks), dim3(kBlock), 0, st, a);
-        if (we) hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(256), 0, st, blocks, s.epart.p, d_term_energy, 0);
-        HIP_OK(hipGetLastError());
+        s.fr0.alloc(N); s.fr1.alloc(N); s.fr2.alloc(N);
+        s.epart.alloc((size_t)blocks_for((size_t)N * kDriverTpp) * 8);
+        pair_driver_launch(term, d_force, d_torque, d_virial, virial_pitch, N, d_pos, d_orientation, d_charge, box,
+                           d_n_neigh, d_nlist, d_head_list, dP, s.fr0.p, s.fr1.p, s.fr2.p, s.epart.p, d_term_energy,
+                           st);
     });
 }
 

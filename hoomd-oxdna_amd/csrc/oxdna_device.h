@@ -506,7 +506,7 @@ __device__ __forceinline__ float coax_pair(V Rc, const Frame& fi, const Frame& f
     return e;
 }
 
-// Debye-Hueckel (reference src/EvaluatorPairOxDNADebye.h:467-526), p = i half.
+// Debye-Hueckel (reference src/EvaluatorPairOxDNADebye.h:85-141), p = i half.
 __device__ __forceinline__ float dh_pair(V Rc, const Frame& fi, const Frame& fj, float qiqj, const DDebye& p_,
                                          V& F, V& T) {
     const DDebye p = eager(p_);

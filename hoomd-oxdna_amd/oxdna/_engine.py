@@ -43,6 +43,7 @@ _SIGS = {
     "oxdna_set_lanes_per_particle": (C.c_int, [_p, C.c_int]),
     "oxdna_get_tuning": (C.c_int, [_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "oxdna_time_forces": (C.c_int, [_p, _u32, _u32, C.POINTER(C.c_double)]),
+    "oxdna_time_steps": (C.c_int, [_p, _u32, _u32, C.POINTER(C.c_double)]),
     "oxdna_count_evaluations": (C.c_int, [_p, _u32, C.POINTER(_u64)]),
     "oxdna_set_replicas": (C.c_int, [_p, _u32, C.POINTER(_u32), _dp]),
     "oxdna_set_replica_bias": (C.c_int, [_p, _ip, _dp, _dp]),

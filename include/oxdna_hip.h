@@ -301,6 +301,13 @@ int oxdna_synchronize(oxdna_ctx* ctx);
 /* Measurement: times nrep launches of the fused force kernel for term_mask at the
  * current state with HIP events on the context's stream; *avg_ms = mean duration. */
 int oxdna_time_forces(oxdna_ctx* ctx, unsigned int term_mask, unsigned int nrep, double* avg_ms);
+/* Measurement: advances nsteps Langevin steps like oxdna_run with use_graph = 1
+ * (nsteps >= 7) and returns in *avg_ms the mean duration of one graph-launched inner
+ * step, timed with HIP events on the context's stream around the graph replays: the
+ * fused force + integrator launch a run step is (on systems too large for that
+ * fusion, the force launch and the integrator kernel). No counterpart in the
+ * reference (measurement of this engine's run step). */
+int oxdna_time_steps(oxdna_ctx* ctx, unsigned int term_mask, unsigned int nsteps, double* avg_ms);
 /* Measurement: directed evaluation counts of the current state, out[2*t] = pairs of
  * term t inside its cut-off, out[2*t+1] = those with nonzero energy (t = bonded:
  * bond endpoints, nonzero = stacking energy != 0). */

@@ -485,7 +485,7 @@ bool eval_coax(const V3<R>& dr, const Quat<R>& qi, const Quat<R>& qj, R rcutsq,
 }
 
 // Debye-Hueckel between BACK sites, times q_i q_j
-// (reference src/EvaluatorPairOxDNADebye.h:467-526).
+// (reference src/EvaluatorPairOxDNADebye.h:85-141).
 template<class R>
 bool eval_debye(const V3<R>& dr, const Quat<R>& qi, const Quat<R>& qj, R rcutsq,
                 const oxdna_debye_params& p, R chg_i, R chg_j, PairOut<R>& o) {
