@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -2614,6 +2615,66 @@ void pair_driver_launch(int term, float* d_force, float* d_torque, float* d_viri
     HIP_OK(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ no-allocation driver
+// oxdna_pair_forces_noalloc runs on one fixed device array instead of scratch the
+// library allocates: the parameter block, three frame arrays of kNoAllocMaxN float4 and
+// the per-workgroup energy partials, 3.3 MiB in all, which each device that runs the
+// library holds until the process ends. Every call on a device shares its copy (the
+// last call's contents stay): calls that overlap on one device race on it.
+constexpr uint32_t kNoAllocMaxN = 65536;
+constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+constexpr size_t kNaParams = 0;
+constexpr size_t kNaFrames = align256(sizeof(DParams));
+constexpr size_t kNaEpart = kNaFrames + align256(3 * (size_t)kNoAllocMaxN * sizeof(float4));
+constexpr size_t kNaBytes =
+    kNaEpart + align256(((size_t)kNoAllocMaxN * kDriverTpp + kBlock - 1) / kBlock * 8 * sizeof(double));
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+// The device address of `scratch` on each device (each holds its own copy), resolved
+// once per device with that device current. Host code uses these, never the array's
+// name (a host-side stand-in).
+unsigned char* g_scratch_ptr[64] = {};
+std::mutex g_scratch_mu;
+
+unsigned char* noalloc_scratch() {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw Fail("device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Fail(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (unsigned char*)p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// The parameter block goes to the device as kernel arguments (two 2 KiB halves), so a
+// call makes no host-to-device copy from pageable memory.
+struct ParamHalf {
+    uint4 w[128];
+};
+static_assert(sizeof(DParams) <= 2 * sizeof(ParamHalf), "the parameter block travels in two halves");
+
+__global__ void k_put_params(ParamHalf h, uint4* dst, uint32_t nwords) {
+    for (uint32_t k = threadIdx.x; k < nwords; k += blockDim.x) dst[k] = h.w[k];
+}
+
+void put_params(const DParams& P, DParams* dP, hipStream_t st) {
+    constexpr uint32_t words = (uint32_t)(sizeof(DParams) / sizeof(uint4));
+    static_assert(sizeof(DParams) % sizeof(uint4) == 0, "DParams is a whole number of 16-byte words");
+    const uint4* src = (const uint4*)&P;
+    for (uint32_t h0 = 0; h0 < words; h0 += 128) {
+        ParamHalf h{};
+        const uint32_t n = std::min<uint32_t>(128u, words - h0);
+        std::memcpy(h.w, src + h0, n * sizeof(uint4));
+        hipLaunchKernelGGL(k_put_params, dim3(1), dim3(128), 0, st, h, (uint4*)dP + h0, n);
+    }
+}
+
 __global__ void k_bond_table(uint32_t N, const uint32_t* blist, const uint32_t* bpos, uint32_t pitch,
                              const uint32_t* n_bonds, int* bpart, int* binfo, uint32_t* err) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2653,6 +2714,31 @@ int oxdna_pair_forces(int term, float* d_force, float* d_torque, float* d_virial
         pair_driver_launch(term, d_force, d_torque, d_virial, virial_pitch, N, d_pos, d_orientation, d_charge, box,
                            d_n_neigh, d_nlist, d_head_list, dP, s.fr0.p, s.fr1.p, s.fr2.p, s.epart.p, d_term_energy,
                            st);
+    });
+}
+
+int oxdna_pair_forces_noalloc(int term, float* d_force, float* d_torque, float* d_virial, size_t virial_pitch,
+                              unsigned int N, const double* d_pos, const double* d_orientation,
+                              const double* d_charge, const oxdna_box* box, const unsigned int* d_n_neigh,
+                              const unsigned int* d_nlist, const size_t* d_head_list, int ntypes,
+                              const void* params, const double* rcutsq, double* d_term_energy, void* stream) {
+    return guard(nullptr, [&] {
+        const DParams P = pair_driver_params("oxdna_pair_forces_noalloc", term, N, d_force, d_torque, d_pos,
+                                             d_orientation, d_charge, box, d_n_neigh, d_nlist, d_head_list, ntypes,
+                                             params, rcutsq);
+        if (N > kNoAllocMaxN)
+            throw Fail("oxdna_pair_forces_noalloc: " + std::to_string(N) + " particles; the fixed scratch holds at most " +
+                       std::to_string(kNoAllocMaxN) + " (use oxdna_pair_forces)");
+        unsigned char* base = noalloc_scratch();  // resolves the array's address on this device once
+        if (N == 0) return;
+        hipStream_t st = (hipStream_t)stream;
+        DParams* dP = (DParams*)(base + kNaParams);
+        put_params(P, dP, st);
+        float4* fr = (float4*)(base + kNaFrames);
+        double* epart = (double*)(base + kNaEpart);
+        pair_driver_launch(term, d_force, d_torque, d_virial, virial_pitch, N, d_pos, d_orientation, d_charge, box,
+                           d_n_neigh, d_nlist, d_head_list, dP, fr, fr + kNoAllocMaxN, fr + 2 * (size_t)kNoAllocMaxN,
+                           epart, d_term_energy, st);
     });
 }
 

@@ -50,6 +50,8 @@ _SIGS = {
     "oxdna_get_replica_energies": (C.c_int, [_p, _dp]),
     "oxdna_pair_forces": (C.c_int, [C.c_int, _p, _p, _p, C.c_size_t, _u32, _p, _p, _p, C.POINTER(_abi.Box),
                                     _p, _p, _p, C.c_int, _p, _dp, _p, _p]),
+    "oxdna_pair_forces_noalloc": (C.c_int, [C.c_int, _p, _p, _p, C.c_size_t, _u32, _p, _p, _p,
+                                            C.POINTER(_abi.Box), _p, _p, _p, C.c_int, _p, _dp, _p, _p]),
     "oxdna_bonded_forces": (C.c_int, [_p, _p, _p, C.c_size_t, _p, _u32, _p, _p, C.POINTER(_abi.Box), _p, _p,
                                       _u32, _p, C.POINTER(_abi.BondedParams), _u32, _p]),
 }

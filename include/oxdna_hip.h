@@ -195,6 +195,26 @@ int oxdna_pair_forces(int term,
                       int ntypes, const void* params, const double* rcutsq,
                       double* d_term_energy, void* stream);
 
+/* oxdna_pair_forces with no device allocation: the same arguments, kernels and
+ * outputs (bit-identical), but its scratch (parameter block, frames, energy
+ * partials) is one fixed device array inside the library instead of memory it
+ * allocates, and the parameter block reaches the device as kernel arguments (no
+ * host-to-device copy). No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Limit: N <= 65536 particles (larger N fails with
+ * an error); the array (3.3 MiB) stays on each device that ran the call until the
+ * process ends. Every call on one device shares that array, so calls that overlap
+ * on one device (other streams or threads) race. */
+int oxdna_pair_forces_noalloc(int term,
+                              float* d_force, float* d_torque, float* d_virial, size_t virial_pitch,
+                              unsigned int N,
+                              const double* d_pos, const double* d_orientation, const double* d_charge,
+                              const oxdna_box* box,
+                              const unsigned int* d_n_neigh, const unsigned int* d_nlist,
+                              const size_t* d_head_list,
+                              int ntypes, const void* params, const double* rcutsq,
+                              double* d_term_energy, void* stream);
+
 /* Bonded force (FENE in FP64 + bonded exc. vol + oxDNA/oxRNA stacking). Replaces
  * gpu_compute_oxdna_bonded_forces (reference src/OxDNABondedGPU.cuh:25-40,
  * src/OxDNABondedGPU.cu:125-159). Bond table in HOOMD's GPU layout: for bond slot
