@@ -42,7 +42,7 @@ __device__ __forceinline__ V site(const float s[3], const Frame& f) {
 // arccos on [-1, 1] by Abramowitz & Stegun 4.4.46: acos(x) = sqrt(1 - x) P7(x) for
 // 0 <= x <= 1 with |error| <= 2e-8 rad, and acos(-x) = pi - acos(x). Also returns
 // sin(acos(x)) = sqrt((1 - x)(1 + x)), which the f4 torque coefficient divides by.
-__device__ __forceinline__ float acos_as(float x, float& sin_out) {
+__host__ __device__ __forceinline__ float acos_as(float x, float& sin_out) {
     float ax = fabsf(x);
     float p = -0.0012624911f;
     p = fmaf(p, ax, 0.0066700901f);
