@@ -287,20 +287,31 @@ __device__ __forceinline__ void merged_particle_pred(const StepArgs& a, uint32_t
     step_finish(a, i, x, q, p, b, charge);
 }
 
-// First half of step t from the stored state (the first step of a run chunk), with the
-// Langevin force and torque of the stored second half.
-__global__ void k_step1(StepArgs a) {
+// The per-particle state a run chunk starts from: what a chunk whose cells or rows
+// overflowed is run again from.
+struct RunState {
+    double4 *pos, *vel, *quat, *angmom, *pos_last, *bd, *bd_tq;
+    float4 *force, *torque;
+};
+
+// First half of step 1 of a run chunk from the stored state (step 0, the chunk's steps
+// numbered from step0), with the Langevin force and torque of the stored second half;
+// each particle's starting state is copied to `save` as it is read.
+__global__ void k_step1(StepArgs a, RunState save, unsigned long long step0) {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i == 0) *a.step_out = *a.step_in + 1ull;
+    if (i == 0) *a.step_out = step0 + 1ull;
     if (i >= a.N) return;
     double4 x = a.pos_in[i], v = a.vel[i], q = a.quat[i], p = a.angmom[i];
     const float4 f = a.force[i], t = a.torque[i];
+    const double4 bd = a.bd[i], bdt = a.bd_tq[i], plast = a.pos_last[i];
+    save.pos[i] = x; save.vel[i] = v; save.quat[i] = q; save.angmom[i] = p;
+    save.force[i] = f; save.torque[i] = t; save.bd[i] = bd; save.bd_tq[i] = bdt; save.pos_last[i] = plast;
     step_clear(a, a.pos_out[i]);
-    lv_first_half_translate(a.lv, a.L, a.Linv, f.x, f.y, f.z, a.bd[i], x, v);
+    lv_first_half_translate(a.lv, a.L, a.Linv, f.x, f.y, f.z, bd, x, v);
     a.pos_out[i] = x;
     a.vel[i] = v;
-    const CellSlot b = step_bin(a, x, a.pos_last[i]);
-    lv_first_half_rotate(a.lv, t.x, t.y, t.z, a.bd_tq[i], q, p);
+    const CellSlot b = step_bin(a, x, plast);
+    lv_first_half_rotate(a.lv, t.x, t.y, t.z, bdt, q, p);
     step_finish(a, i, x, q, p, b, a.charge ? (float)a.charge[i] : 1.f);
 }
 
@@ -865,31 +876,14 @@ __global__ void k_reduce_energy(uint32_t nblocks, const double* __restrict__ par
     }
 }
 
-// The per-particle state a run chunk starts from: what a chunk whose cells or rows
-// overflowed is run again from.
-struct RunState {
-    double4 *pos, *vel, *quat, *angmom, *pos_last, *bd, *bd_tq;
-    float4 *force, *torque;
-};
-
-// Copies `from` into `to` (N particles). Start of a run chunk (f != NULL): also zeroes
-// the ncount cell counts and numbers the chunk's steps from step0, with no rebuild
-// request pending and the overflow record cleared.
-__global__ void k_chunk_begin(uint32_t N, RunState from, RunState to, uint32_t* counts, size_t ncount, DevFlags* f,
-                              unsigned long long step0) {
+// Copies the N particle states `from` into `to` (a run chunk's restore).
+__global__ void k_copy_state(uint32_t N, RunState from, RunState to) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
     for (size_t k = g; k < N; k += stride) {
         to.pos[k] = from.pos[k]; to.vel[k] = from.vel[k]; to.quat[k] = from.quat[k];
         to.angmom[k] = from.angmom[k]; to.pos_last[k] = from.pos_last[k];
         to.bd[k] = from.bd[k]; to.bd_tq[k] = from.bd_tq[k];
         to.force[k] = from.force[k]; to.torque[k] = from.torque[k];
-    }
-    for (size_t k = g; k < ncount; k += stride) counts[k] = 0u;
-    if (f && g == 0) {
-        f->stepc[0] = step0;
-        f->rebuild[0] = f->rebuild[1] = f->rebuild[2] = 0u;
-        f->overflow = 0u; f->max_cell = 0u; f->max_neigh = 0u;
-        f->step = step0;
     }
 }
 
@@ -900,10 +894,12 @@ struct RunStatus {
 };
 
 // End of a run chunk: its state is in buffer `odd` of the position and frame pairs;
-// bring it to buffer 0 (where every other entry point reads it), record the step count
-// and report the overflow record to the host.
-__global__ void k_chunk_end(uint32_t N, int odd, double4* pos, float4* fr0, float4* fr1, float4* fr2, DevFlags* f,
-                            unsigned long long step_end, RunStatus* out) {
+// bring it to buffer 0 (where every other entry point reads it), record the step count,
+// report the overflow record to the host, and leave what the next chunk's step 1
+// expects: the ncount cell counts zero, no rebuild request or overflow pending.
+__global__ void k_chunk_end(uint32_t N, int odd, double4* pos, float4* fr0, float4* fr1, float4* fr2,
+                            uint32_t* counts, size_t ncount, DevFlags* f, unsigned long long step_end,
+                            RunStatus* out) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
     if (odd) {
         for (size_t k = g; k < N; k += stride) {
@@ -911,6 +907,7 @@ __global__ void k_chunk_end(uint32_t N, int odd, double4* pos, float4* fr0, floa
             fr0[k] = fr0[N + k]; fr1[k] = fr1[N + k]; fr2[k] = fr2[N + k];
         }
     }
+    for (size_t k = g; k < ncount; k += stride) counts[k] = 0u;
     if (g == 0) {
         f->step = step_end;
         out->overflow = f->overflow;
@@ -918,6 +915,8 @@ __global__ void k_chunk_end(uint32_t N, int odd, double4* pos, float4* fr0, floa
         out->max_neigh = f->max_neigh;
         out->builds = f->builds;
         out->done = 1u;
+        f->rebuild[0] = f->rebuild[1] = f->rebuild[2] = 0u;
+        f->overflow = 0u; f->max_cell = 0u; f->max_neigh = 0u;
     }
 }
 
@@ -1013,7 +1012,6 @@ struct oxdna_ctx {
     int fixed_tpp = 0;                   // nonzero: lanes per particle pinned by the caller
     uint32_t forces_mask = 0xffffffffu;  // mask of the forces currently in `force`
     hipGraphExec_t graph = nullptr;      // kGraphSteps inner steps
-    hipGraphExec_t graph6 = nullptr;     // 6 inner steps (the tail of a chunk)
     uint32_t graph_mask = 0;
     uint64_t host_step = 0;
     // the state a run chunk starts from (restored when the chunk must run again)
@@ -1041,7 +1039,6 @@ struct oxdna_ctx {
 
     ~oxdna_ctx() {
         if (graph) (void)hipGraphExecDestroy(graph);
-        if (graph6) (void)hipGraphExecDestroy(graph6);
         sv_pos.release(); sv_vel.release(); sv_quat.release(); sv_angmom.release(); sv_pos_last.release();
         sv_bd.release(); sv_bd_tq.release(); sv_force.release(); sv_torque.release();
         if (status_h) (void)hipHostFree(status_h);
@@ -1060,9 +1057,7 @@ struct oxdna_ctx {
 
     void drop_graph() {
         if (graph) (void)hipGraphExecDestroy(graph);
-        if (graph6) (void)hipGraphExecDestroy(graph6);
         graph = nullptr;
-        graph6 = nullptr;
     }
 };
 
@@ -1602,10 +1597,10 @@ void enqueue_last(oxdna_ctx* c, uint32_t mask, uint64_t k) {
 }
 
 // Inner steps per graph: a multiple of 6, so that its buffer indices (step mod 2 and
-// mod 3) are the same at every replay. A chunk replays the long graph, then the short
-// one, and launches the last (< 6) inner steps directly.
+// mod 3) are the same at every replay. A chunk replays the graph, then launches the
+// remaining (< 96) inner steps directly (measured: for short chunks direct launches
+// beat graphs of 6 steps, whose start and end add gaps to the stream).
 constexpr uint32_t kGraphSteps = 96;
-constexpr uint32_t kGraphStepsShort = 6;
 
 hipGraphExec_t capture_inner(oxdna_ctx* c, uint32_t mask, uint32_t len) {
     hipGraph_t g;
@@ -1622,7 +1617,6 @@ hipGraphExec_t capture_inner(oxdna_ctx* c, uint32_t mask, uint32_t len) {
 void build_graph(oxdna_ctx* c, uint32_t mask) {
     c->drop_graph();
     c->graph = capture_inner(c, mask, kGraphSteps);
-    c->graph6 = capture_inner(c, mask, kGraphStepsShort);
     c->graph_mask = mask;
 }
 
@@ -1683,11 +1677,13 @@ RunState saved_state(oxdna_ctx* c) {
 
 uint32_t grid_stride_blocks(size_t n) { return (uint32_t)std::max<size_t>(1, std::min<size_t>(blocks_for(n), 2048)); }
 
-// Runs `nsteps` Langevin steps in chunks of up to 42 graph lengths. A chunk costs the
-// host two launches of its own (k_chunk_begin: state saved, cells emptied, steps
-// numbered; k_chunk_end: state to buffer 0, status into pinned host memory) and one
-// stream synchronisation; a chunk whose cells or rows overflowed is run again from
-// the saved start with larger capacities.
+// Runs `nsteps` Langevin steps in chunks of up to 42 graph lengths. Between runs (and
+// after every single-point evaluation) the cell counts are zero and no rebuild request
+// or overflow is pending, so a chunk is: k_step1 (which also saves the chunk's starting
+// state), the inner steps, the last step, and k_chunk_end (state to buffer 0, status
+// into pinned host memory, cells and flags left clean), then one stream
+// synchronisation. A chunk whose cells or rows overflowed is run again from the saved
+// start with larger capacities.
 void run(oxdna_ctx* c, uint32_t nsteps, uint32_t mask, int use_graph) {
     if (!c->have_langevin) throw Fail("no integrator: call oxdna_set_langevin first");
     ensure_ready(c);
@@ -1702,32 +1698,28 @@ void run(oxdna_ctx* c, uint32_t nsteps, uint32_t mask, int use_graph) {
     uint32_t done = 0;
     while (done < nsteps) {
         const uint32_t chunk = std::min(chunk_max, nsteps - done);
-        // the chunk's steps are numbered 1 ... chunk from its starting state (step 0):
-        // random-stream counter, empty cells, no rebuild requests
+        // the chunk's steps are numbered 1 ... chunk from its starting state (step 0)
         const unsigned long long step0 = c->host_step + done;
         for (int attempt = 0;; ++attempt) {
             const size_t ncount = pairs ? 3 * ncell_total(c) : 0;
-            hipLaunchKernelGGL(k_chunk_begin, dim3(grid_stride_blocks(std::max<size_t>(c->N, ncount))), dim3(kBlock),
-                               0, c->stream, c->N, live_state(c), saved_state(c), c->cell_count.p, ncount, c->flags.p,
-                               step0);
-            hipLaunchKernelGGL(k_step1, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, step_args(c, pairs, 1));
+            hipLaunchKernelGGL(k_step1, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, step_args(c, pairs, 1),
+                               saved_state(c), step0);
             uint32_t k = 0;  // inner steps enqueued
-            const bool timed = c->timing && use_graph && chunk - 1 >= kGraphStepsShort;
-            if (use_graph && chunk - 1 >= kGraphStepsShort) {
+            const bool graphs = use_graph && chunk - 1 >= kGraphSteps;
+            const bool timed = c->timing && graphs;
+            if (graphs) {
                 if (!c->graph || c->graph_mask != m) build_graph(c, m);
                 if (timed) HIP_OK(hipEventRecord(c->ev0, c->stream));
                 for (; k + kGraphSteps <= chunk - 1; k += kGraphSteps) HIP_OK(hipGraphLaunch(c->graph, c->stream));
-                for (; k + kGraphStepsShort <= chunk - 1; k += kGraphStepsShort)
-                    HIP_OK(hipGraphLaunch(c->graph6, c->stream));
                 if (timed) HIP_OK(hipEventRecord(c->ev1, c->stream));
             }
             // the rest without the graph: the same launches, so both paths are bit-identical
             enqueue_inner(c, m, k, chunk - 1 - k);
             enqueue_last(c, m, chunk);
             c->status_h->done = 0u;
-            hipLaunchKernelGGL(k_chunk_end, dim3(grid_stride_blocks(c->N)), dim3(kBlock), 0, c->stream, c->N,
-                               (int)(chunk & 1u), c->pos.p, c->fr0.p, c->fr1.p, c->fr2.p, c->flags.p,
-                               step0 + chunk, c->status_d);
+            hipLaunchKernelGGL(k_chunk_end, dim3(grid_stride_blocks(std::max<size_t>(c->N, ncount))), dim3(kBlock), 0,
+                               c->stream, c->N, (int)(chunk & 1u), c->pos.p, c->fr0.p, c->fr1.p, c->fr2.p,
+                               c->cell_count.p, ncount, c->flags.p, step0 + chunk, c->status_d);
             HIP_OK(hipGetLastError());
             HIP_OK(hipStreamSynchronize(c->stream));
             const volatile RunStatus* st = c->status_h;
@@ -1740,10 +1732,11 @@ void run(oxdna_ctx* c, uint32_t nsteps, uint32_t mask, int use_graph) {
             }
             if (st->overflow == 0u) break;
             if (attempt >= 4) throw Fail("neighbour list kept overflowing during the run");
-            // grow, restore the chunk's starting state and run it again
+            // grow, restore the chunk's starting state and run it again (setup_cells and
+            // compute leave the cells empty and the flags clean)
             const uint32_t max_neigh = st->max_neigh, max_cell = st->max_cell;
-            hipLaunchKernelGGL(k_chunk_begin, dim3(grid_stride_blocks(c->N)), dim3(kBlock), 0, c->stream, c->N,
-                               saved_state(c), live_state(c), (uint32_t*)nullptr, (size_t)0, (DevFlags*)nullptr, 0ull);
+            hipLaunchKernelGGL(k_copy_state, dim3(grid_stride_blocks(c->N)), dim3(kBlock), 0, c->stream, c->N,
+                               saved_state(c), live_state(c));
             c->max_neigh = std::max(c->max_neigh, (uint32_t)std::ceil(max_neigh * 1.5) + 8);
             c->cell_cap = std::max(c->cell_cap, (uint32_t)std::ceil(max_cell * 1.5) + 4);
             c->drop_graph();
@@ -2422,8 +2415,8 @@ int oxdna_time_steps(oxdna_ctx* c, unsigned int mask, unsigned int nsteps, doubl
     return guard(c, [&] {
         if (!avg_ms) throw Fail("oxdna_time_steps: avg_ms is NULL");
         if (c->n_rep) throw Fail("oxdna_time_steps: one-system contexts only (replicas: oxdna_time_forces)");
-        if (nsteps < 1 + kGraphStepsShort)
-            throw Fail("oxdna_time_steps: nsteps must be at least " + std::to_string(1 + kGraphStepsShort));
+        if (nsteps < 1 + kGraphSteps)
+            throw Fail("oxdna_time_steps: nsteps must be at least " + std::to_string(1 + kGraphSteps));
         if (!c->ev0) HIP_OK(hipEventCreate(&c->ev0));
         if (!c->ev1) HIP_OK(hipEventCreate(&c->ev1));
         c->timed_ms = 0.0;

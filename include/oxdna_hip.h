@@ -322,7 +322,7 @@ int oxdna_synchronize(oxdna_ctx* ctx);
  * current state with HIP events on the context's stream; *avg_ms = mean duration. */
 int oxdna_time_forces(oxdna_ctx* ctx, unsigned int term_mask, unsigned int nrep, double* avg_ms);
 /* Measurement: advances nsteps Langevin steps like oxdna_run with use_graph = 1
- * (nsteps >= 7) and returns in *avg_ms the mean duration of one graph-launched inner
+ * (nsteps >= 97) and returns in *avg_ms the mean duration of one graph-launched inner
  * step, timed with HIP events on the context's stream around the graph replays: the
  * fused force + integrator launch a run step is (on systems too large for that
  * fusion, the force launch and the integrator kernel). No counterpart in the

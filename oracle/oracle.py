@@ -32,20 +32,37 @@ def build():
     subprocess.check_call(["make", "-s", "-C", HERE])
 
 
+def _load(path):
+    L = C.CDLL(path)
+    L.oracle_compute.argtypes = [C.POINTER(OracleSystem), C.c_uint32, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.oracle_pairs.argtypes = [C.POINTER(OracleSystem), C.c_double, C.c_int, _ip, C.c_size_t,
+                               C.POINTER(C.c_size_t)]
+    L.oracle_pair.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_void_p, C.c_double, C.c_double, _dp]
+    L.oracle_bond.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_int, _dp]
+    L.oracle_census.argtypes = [C.POINTER(OracleSystem), C.c_uint32, _dp]
+    L.oracle_compute_list.argtypes = [C.POINTER(OracleSystem), C.c_uint32, _ip, C.c_size_t, C.c_int, _dp, _dp,
+                                      _dp]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        L.oracle_compute.argtypes = [C.POINTER(OracleSystem), C.c_uint32, C.c_int, _dp, _dp, _dp, _dp, _dp]
-        L.oracle_pairs.argtypes = [C.POINTER(OracleSystem), C.c_double, C.c_int, _ip, C.c_size_t,
-                                   C.POINTER(C.c_size_t)]
-        L.oracle_pair.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_void_p, C.c_double, C.c_double, _dp]
-        L.oracle_bond.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_int, _dp]
-        L.oracle_census.argtypes = [C.POINTER(OracleSystem), C.c_uint32, _dp]
-        _lib = L
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+def baseline_lib(out_dir):
+    """The oracle compiled for timing on this host (bench.py's cpu_baseline leg):
+    g++ -O3 -march=native with OpenMP, into out_dir. Returns (library, compile command)."""
+    path = os.path.join(out_dir, "liboxdna_oracle_native.so")
+    cmd = ["g++", "-O3", "-march=native", "-fopenmp", "-std=c++17", "-fPIC", "-ffp-contract=off",
+           "-I" + os.path.join(HERE, "..", "include"), "-shared", os.path.join(HERE, "oxdna_oracle.cpp"), "-o", path]
+    subprocess.check_call(cmd)
+    shown = cmd[:7] + ["-Iinclude", "-shared", "oracle/oxdna_oracle.cpp"]
+    return _load(path), " ".join(shown)
 
 
 def _arr(a, dtype):
@@ -117,6 +134,19 @@ class Oracle:
                                   E.ctypes.data_as(_dp), V.ctypes.data_as(_dp))
         assert rc == 0
         return dict(F=F, T=T, Eh=Eh, E=E, V=V)
+
+    def compute_list(self, pairs, nthreads=1, mask=None, library=None):
+        """Forces, torques and per-term energies over a given pair list (i < j) and the
+        bonds on nthreads threads: dict(F, T, E)."""
+        n = len(self.pos)
+        F, T, E = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(8)
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32)
+        L = library if library is not None else lib()
+        rc = L.oracle_compute_list(C.byref(self.sys), self.mask if mask is None else mask,
+                                   pairs.ctypes.data_as(_ip), len(pairs), int(nthreads),
+                                   F.ctypes.data_as(_dp), T.ctypes.data_as(_dp), E.ctypes.data_as(_dp))
+        assert rc == 0
+        return dict(F=F, T=T, E=E)
 
     def pairs(self, r_list, use_cells=True):
         cap = max(16, len(self.pos) * 128)

@@ -18,7 +18,12 @@
 //     (reference src/OxDNABondedForceCompute.cc:76-131).
 #include "oxdna_oracle.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <set>
 #include <utility>
@@ -256,6 +261,74 @@ int oracle_compute(const oracle_system* s, uint32_t mask, int use_cells, double*
     if (Eh) std::memcpy(Eh, eh.data(), sizeof(double) * n);
     if (Vir) std::memcpy(Vir, vir.data(), sizeof(double) * 6 * n);
     if (Eterm) std::memcpy(Eterm, et, sizeof(et));
+    return 0;
+}
+
+// CPU baseline (bench.py's cpu_baseline leg): forces, torques and per-term energies of
+// the terms in mask over a given pair list (i < j, e.g. from oracle_pairs with a skin,
+// reused between list rebuilds as an MD code does) and the bonds, on nthreads OpenMP
+// threads (each with its own accumulators, summed at the end; serial without OpenMP).
+// F[3n], T[3n], Eterm[8] are overwritten; any may be NULL.
+int oracle_compute_list(const oracle_system* s, uint32_t mask, const int32_t* pairs, size_t npairs, int nthreads,
+                        double* F, double* T, double* Eterm) {
+    const uint32_t n = s->n;
+    int nt = nthreads > 0 ? nthreads : 1;
+#ifndef _OPENMP
+    nt = 1;
+#endif
+    std::vector<std::vector<double>> fa(nt, std::vector<double>(3 * n, 0.0)), ta(nt, std::vector<double>(3 * n, 0.0));
+    std::vector<std::array<double, 8>> ea(nt);
+    for (auto& e : ea) e.fill(0.0);
+    const bool bonded = ((mask >> OXDNA_TERM_BONDED) & 1u) && s->bond_params;
+#pragma omp parallel num_threads(nt)
+    {
+#ifdef _OPENMP
+        const int me = omp_get_thread_num();
+#else
+        const int me = 0;
+#endif
+        double* f = fa[me].data();
+        double* t = ta[me].data();
+        double* et = ea[me].data();
+#pragma omp for schedule(dynamic, 256) nowait
+        for (long long k = 0; k < (long long)npairs; ++k) {
+            const int i = pairs[2 * k], j = pairs[2 * k + 1];
+            double dr[3];
+            rsq_of(s, i, j, dr);
+            const V3<double> d = V3<double>::from(dr);
+            for (int term = 0; term < 6; ++term) {
+                if (!((mask >> term) & 1u) || !s->pair_params[term]) continue;
+                PairOut<double> o;
+                if (!eval_term(term, s, i, j, d, o)) continue;
+                et[kSlot[term]] += o.E;
+                if (term == OXDNA_TERM_HBONDCROSS) et[OXDNA_E_CRSTACK] += o.E2;
+                f[3 * i + 0] += o.F.x; f[3 * i + 1] += o.F.y; f[3 * i + 2] += o.F.z;
+                f[3 * j + 0] -= o.F.x; f[3 * j + 1] -= o.F.y; f[3 * j + 2] -= o.F.z;
+                t[3 * i + 0] += o.Ti.x; t[3 * i + 1] += o.Ti.y; t[3 * i + 2] += o.Ti.z;
+                t[3 * j + 0] += o.Tj.x; t[3 * j + 1] += o.Tj.y; t[3 * j + 2] += o.Tj.z;
+            }
+        }
+        if (bonded) {
+#pragma omp for schedule(static) nowait
+            for (long long b = 0; b < (long long)s->nb; ++b) {
+                const int A = s->bonds[2 * b], B = s->bonds[2 * b + 1];
+                BondOut<double> r;
+                bond_eval(s, (uint32_t)b, r);
+                et[OXDNA_E_FENE] += r.e_fene; et[OXDNA_E_BEXC] += r.e_bexc; et[OXDNA_E_STACK] += r.e_stack;
+                f[3 * A + 0] -= r.FB.x; f[3 * A + 1] -= r.FB.y; f[3 * A + 2] -= r.FB.z;
+                f[3 * B + 0] += r.FB.x; f[3 * B + 1] += r.FB.y; f[3 * B + 2] += r.FB.z;
+                t[3 * A + 0] += r.TA.x; t[3 * A + 1] += r.TA.y; t[3 * A + 2] += r.TA.z;
+                t[3 * B + 0] += r.TB.x; t[3 * B + 1] += r.TB.y; t[3 * B + 2] += r.TB.z;
+            }
+        }
+    }
+    for (int w = 1; w < nt; ++w) {
+        for (size_t k = 0; k < 3 * (size_t)n; ++k) { fa[0][k] += fa[w][k]; ta[0][k] += ta[w][k]; }
+        for (int k = 0; k < 8; ++k) ea[0][k] += ea[w][k];
+    }
+    if (F) std::memcpy(F, fa[0].data(), sizeof(double) * 3 * n);
+    if (T) std::memcpy(T, ta[0].data(), sizeof(double) * 3 * n);
+    if (Eterm) std::memcpy(Eterm, ea[0].data(), sizeof(double) * 8);
     return 0;
 }
 
