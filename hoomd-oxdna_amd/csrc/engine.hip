@@ -443,7 +443,12 @@ __device__ __forceinline__ uint32_t prefetch_args() {
 // spills: large systems are bound by issue rate across many waves); the fused one, used
 // only while the grid holds <= 2 waves per SIMD, and the single-point energy + virial
 // one, unconstrained.
-template<int P, bool WANT_E, bool WANT_V, bool COUNT, bool FUSED = false, bool FAR2 = false>
+// MULTI: some term's table differs between type pairs (a.xwords != 0): each pair finds its
+// parameter sets through their offsets in the block, staged with the extras in dynamic
+// LDS. Otherwise the sets sit at fixed offsets of a static copy of DParams, where the
+// compiler merges their reads into 64- and 128-bit LDS loads (a set address chosen at run
+// time loses that: measured 3 % slower).
+template<int P, bool WANT_E, bool WANT_V, bool COUNT, bool FUSED = false, bool FAR2 = false, bool MULTI = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((FUSED || (WANT_E && WANT_V)) ? 1 : 3)))
 void k_forces(ForceArgs a) {
     constexpr uint32_t TPP = 256 / P;
@@ -473,16 +478,26 @@ void k_forces(ForceArgs a) {
     // The term parameters, staged in LDS: the inlined term evaluators reach them through
     // addresses the compiler may select per lane (one code path serving the hydrogen-
     // bonding and cross-stacking halves), which from global memory would be a vector
-    // load, with its latency, per parameter.
-    __shared__ uint4 s_par[(sizeof(DParams) + 15) / 16];
-    extern __shared__ uint32_t lds_rows[];  // rebuild launches: 2 rows of max_neigh per particle
+    // load, with its latency, per parameter. Uniform tables: DParams in s_par; MULTI: the
+    // whole parameter block (DParams, then a.xwords 16-byte words of per-type-pair sets)
+    // at the start of the dynamic LDS. After it, in rebuild launches, 2 rows of max_neigh
+    // per particle.
+    constexpr uint32_t kParWords = (uint32_t)(sizeof(DParams) / 16);
+    __shared__ uint4 s_par[MULTI ? 1 : kParWords];
+    extern __shared__ uint4 lds_dyn[];
+    const uint32_t pwords = MULTI ? kParWords + a.xwords : 0u;
+    uint32_t* lds_rows = reinterpret_cast<uint32_t*>(lds_dyn + pwords);
 
     const uint32_t t = threadIdx.x;
     const uint32_t slot = t / TPP, lane = t % TPP;
     const uint32_t i = xcd_block() * P + slot;
     const bool active = i < a.N;
-    for (uint32_t w = t; w < (sizeof(DParams) + 15) / 16; w += 256) s_par[w] = ((const uint4*)a.P)[w];
-    const DParams& P_ = *(const DParams*)s_par;
+    if constexpr (MULTI) {
+        for (uint32_t w = t; w < pwords; w += 256) lds_dyn[w] = ((const uint4*)a.P)[w];
+    } else {
+        for (uint32_t w = t; w < kParWords; w += 256) s_par[w] = ((const uint4*)a.P)[w];
+    }
+    const DParams& P_ = *reinterpret_cast<const DParams*>(MULTI ? lds_dyn : s_par);
     const uint32_t mask = a.mask;
     const NbrArgs& nb = a.nb;
     constexpr uint32_t fpk = FAR2 ? 2u : 1u;  // far row entries per thread
@@ -703,7 +718,7 @@ void k_forces(ForceArgs a) {
                       (float)min_image(ps.z - pj.z, a.L[2], a.Linv[2]));
             const float rsq = dot(dr, dr);
             V Fp = zero3(), Tp = zero3();
-            const float ep = pair_terms<COUNT>(P_, mask, tp, rsq, -dr, fi, fj, f0.w * g0.w, Fp, Tp, et, cnt);
+            const float ep = pair_terms<COUNT, MULTI>(P_, mask, tp, rsq, -dr, fi, fj, f0.w * g0.w, Fp, Tp, et, cnt);
             float x[NREC];
 #pragma unroll
             for (int q = 0; q < NREC; ++q) x[q] = 0.f;
@@ -984,12 +999,14 @@ struct oxdna_ctx {
     DBuf<size_t> head;
     DBuf<double> epart, energy;
     DBuf<DevFlags> flags;
-    DBuf<DParams> dparams;
+    DBuf<uint4> dparams;                 // DParams, then the extras block (16-byte words)
 
     std::vector<int32_t> bonds_h, btype_h;
     bool have_bonds = false;
 
     DParams P{};
+    std::vector<unsigned char> xset[6];  // per term: its distinct parameter sets beyond the base set
+    uint8_t pidx[6][16] = {};            // per term, per type pair: its set index (0: base set)
     bool pair_set[6] = {false, false, false, false, false, false};
     double rcut_max[6] = {0, 0, 0, 0, 0, 0};
     bool bond_set = false;
@@ -1097,23 +1114,6 @@ F4P cvt_f4(const double s[5]) {
 }
 F5P cvt_f5(const double s[4]) { return F5P{(float)s[0], (float)s[1], (float)s[2], (float)s[3]}; }
 
-template<class T> void require_uniform(const T* tab, int nt, const char* term, T (*clear)(T)) {
-    T ref = clear(tab[0]);
-    for (int k = 1; k < nt * nt; ++k) {
-        T x = clear(tab[k]);
-        if (std::memcmp(&ref, &x, sizeof(T)) != 0)
-            throw Fail(std::string(term) + ": parameters other than the per-type-pair HB epsilon must be the "
-                       "same for every type pair");
-    }
-}
-
-oxdna_excvol_params keep_ev(oxdna_excvol_params p) { return p; }
-oxdna_hbond_params clear_hb(oxdna_hbond_params p) { p.eps = 0; return p; }
-oxdna_crossstacking_params clear_cs(oxdna_crossstacking_params p) { p._pad = 0; return p; }
-oxdna_hbondcross_params clear_hbx(oxdna_hbondcross_params p) { p.eps = 0; p._pad = 0; return p; }
-oxdna_coaxstacking_params clear_cx(oxdna_coaxstacking_params p) { p._pad = 0; return p; }
-oxdna_debye_params keep_dh(oxdna_debye_params p) { return p; }
-
 void set_f1(F1P& d, double a, double r0, double rlow, double rhigh, double rclow, double rchigh, double blow,
             double bhigh) {
     d = F1P{(float)a, (float)r0, (float)rlow, (float)rhigh, (float)rclow, (float)rchigh, (float)blow, (float)bhigh};
@@ -1124,92 +1124,168 @@ void set_f2(F2P& d, double k, double r0, double rc, double rlow, double rhigh, d
             (float)bhigh};
 }
 
-void load_pair_params(DParams& P, int term, int nt, const void* params) {
-    switch (term) {
-    case OXDNA_TERM_EXCVOL: {
-        auto* t = (const oxdna_excvol_params*)params;
-        require_uniform(t, nt, "excluded volume", keep_ev);
-        DExcVol& d = P.ev;
-        d.eps = (float)t[0].eps;
-        cvt3(d.base, t[0].base_site); cvt3(d.back, t[0].back_site);
-        for (int k = 0; k < 4; ++k) {
-            d.sigma[k] = (float)t[0].sigma[k]; d.rstar[k] = (float)t[0].rstar[k];
-            d.b[k] = (float)t[0].b[k]; d.rc[k] = (float)t[0].rc[k];
-        }
-        break;
+// One type pair's parameter struct (C ABI, FP64) as the kernels' FP32 set. The
+// hydrogen-bonding epsilon is left out of the base-base sets (it has a per-type-pair
+// table of its own).
+DExcVol cvt_set(const oxdna_excvol_params& t) {
+    DExcVol d;
+    std::memset(&d, 0, sizeof(d));
+    d.eps = (float)t.eps;
+    cvt3(d.base, t.base_site); cvt3(d.back, t.back_site);
+    for (int k = 0; k < 4; ++k) {
+        d.sigma[k] = (float)t.sigma[k]; d.rstar[k] = (float)t.rstar[k];
+        d.b[k] = (float)t.b[k]; d.rc[k] = (float)t.rc[k];
     }
+    return d;
+}
+DBaseBase cvt_set(const oxdna_hbond_params& t) {
+    DBaseBase d;
+    std::memset(&d, 0, sizeof(d));
+    cvt3(d.base, t.base_site);
+    set_f1(d.f1, t.f1_a, t.f1_r0, t.f1_rlow, t.f1_rhigh, t.f1_rclow, t.f1_rchigh, t.f1_blow, t.f1_bhigh);
+    d.shift_factor = (float)t.shift_factor;
+    d.hb_t1 = cvt_f4(t.t1); d.hb_t23 = cvt_f4(t.t23); d.hb_t4 = cvt_f4(t.t4); d.hb_t78 = cvt_f4(t.t78);
+    return d;
+}
+DBaseBase cvt_set(const oxdna_crossstacking_params& t) {
+    DBaseBase d;
+    std::memset(&d, 0, sizeof(d));
+    cvt3(d.base, t.base_site);
+    set_f2(d.f2, t.f2_k, t.f2_r0, t.f2_rc, t.f2_rlow, t.f2_rhigh, t.f2_rclow, t.f2_rchigh, t.f2_blow, t.f2_bhigh);
+    d.cs_t1 = cvt_f4(t.t1); d.cs_t23 = cvt_f4(t.t23); d.cs_t4 = cvt_f4(t.t4); d.cs_t78 = cvt_f4(t.t78);
+    d.cs_t4_enabled = t.t4_enabled;
+    return d;
+}
+DBaseBase cvt_set(const oxdna_hbondcross_params& t) {
+    DBaseBase d;
+    std::memset(&d, 0, sizeof(d));
+    cvt3(d.base, t.base_site);
+    set_f1(d.f1, t.f1_a, t.f1_r0, t.f1_rlow, t.f1_rhigh, t.f1_rclow, t.f1_rchigh, t.f1_blow, t.f1_bhigh);
+    d.shift_factor = (float)t.shift_factor;
+    d.hb_t1 = cvt_f4(t.hb_t1); d.hb_t23 = cvt_f4(t.hb_t23); d.hb_t4 = cvt_f4(t.hb_t4); d.hb_t78 = cvt_f4(t.hb_t78);
+    set_f2(d.f2, t.f2_k, t.f2_r0, t.f2_rc, t.f2_rlow, t.f2_rhigh, t.f2_rclow, t.f2_rchigh, t.f2_blow, t.f2_bhigh);
+    d.cs_t1 = cvt_f4(t.cs_t1); d.cs_t23 = cvt_f4(t.cs_t23); d.cs_t4 = cvt_f4(t.cs_t4); d.cs_t78 = cvt_f4(t.cs_t78);
+    d.cs_t4_enabled = t.cs_t4_enabled;
+    return d;
+}
+DCoax cvt_set(const oxdna_coaxstacking_params& t) {
+    DCoax d;
+    std::memset(&d, 0, sizeof(d));
+    cvt3(d.stack, t.stack_site); cvt3(d.backref, t.stack_back_ref);
+    d.gamma = (float)t.gamma;
+    set_f2(d.f2, t.f2_k, t.f2_r0, t.f2_rc, t.f2_rlow, t.f2_rhigh, t.f2_rclow, t.f2_rchigh, t.f2_blow, t.f2_bhigh);
+    d.t1 = cvt_f4(t.t1); d.t4 = cvt_f4(t.t4); d.t56 = cvt_f4(t.t56);
+    d.phi3 = cvt_f5(t.phi3); d.phi4 = cvt_f5(t.phi4);
+    d.t1_mode = t.t1_mode; d.phi3_enabled = t.phi3_enabled; d.rna_coax = t.rna_coax;
+    d.t1_sa = (float)t.t1_sa; d.t1_sb = (float)t.t1_sb;
+    if (d.t1_mode != 0 && d.t1_mode != 1) throw Fail("coaxial stacking: t1_mode must be 0 or 1");
+    // theta1's second piece: mode 0 f4(2 pi - theta1), nonzero for theta1 > 2 pi - t0 - tc
+    // (theta1 <= pi); mode 1 SA (theta1 - SB)^2, nonzero for theta1 > SB
+    const double t1_lo = d.t1_mode == 0 ? 2.0 * M_PI - t.t1[2] - t.t1[4] : t.t1_sb;  // (a, b, t0, ts, tc)
+    d.t1_cx = t1_lo >= M_PI ? -2.0f : (t1_lo <= 0.0 ? 2.0f : (float)std::cos(t1_lo));
+    return d;
+}
+DDebye cvt_set(const oxdna_debye_params& t) {
+    DDebye d;
+    std::memset(&d, 0, sizeof(d));
+    cvt3(d.back, t.back_site);
+    d.minus_kappa = (float)t.minus_kappa; d.prefactor = (float)t.prefactor;
+    d.b_smooth = (float)t.b_smooth; d.rc = (float)t.rc; d.rhigh = (float)t.rhigh;
+    return d;
+}
+
+// Distinct parameter sets of one term's ntypes x ntypes table: type pair 0's set is the
+// base set, every other distinct set is appended to `extra` (bytes), and idx[tp]
+// receives each type pair's set index (type pairs beyond the table: 0).
+template<class S, class D>
+void assign_sets(std::vector<unsigned char>& extra, uint8_t idx[16], int nt, const S* tab, D& base) {
+    std::vector<D> sets;
+    for (int k = 0; k < 16; ++k) {
+        if (k >= nt * nt) {
+            idx[k] = 0;
+            continue;
+        }
+        const D d = cvt_set(tab[k]);
+        size_t j = 0;
+        while (j < sets.size() && std::memcmp(&sets[j], &d, sizeof(D)) != 0) ++j;
+        if (j == sets.size()) sets.push_back(d);
+        idx[k] = (uint8_t)j;
+    }
+    base = sets[0];
+    extra.assign(reinterpret_cast<const unsigned char*>(sets.data() + 1),
+                 reinterpret_cast<const unsigned char*>(sets.data() + sets.size()));
+}
+
+// Converts one term's table into P (base set and, for the hydrogen-bonding terms, the
+// per-type-pair epsilon), `extra` (the term's further distinct sets, bytes; empty when
+// the table is uniform) and idx (each type pair's set index).
+void load_pair_params(DParams& P, std::vector<unsigned char>& extra, uint8_t idx[16], int term, int nt,
+                      const void* params) {
+    switch (term) {
+    case OXDNA_TERM_EXCVOL:
+        assign_sets(extra, idx, nt, (const oxdna_excvol_params*)params, P.ev);
+        break;
     case OXDNA_TERM_HBOND: {
         auto* t = (const oxdna_hbond_params*)params;
-        require_uniform(t, nt, "hydrogen bonding", clear_hb);
-        DBaseBase& d = P.hb;
-        cvt3(d.base, t[0].base_site);
-        set_f1(d.f1, t[0].f1_a, t[0].f1_r0, t[0].f1_rlow, t[0].f1_rhigh, t[0].f1_rclow, t[0].f1_rchigh,
-               t[0].f1_blow, t[0].f1_bhigh);
-        d.shift_factor = (float)t[0].shift_factor;
-        for (int k = 0; k < 16; ++k) d.hb_eps[k] = k < nt * nt ? (float)t[k].eps : 0.f;
-        d.hb_t1 = cvt_f4(t[0].t1); d.hb_t23 = cvt_f4(t[0].t23); d.hb_t4 = cvt_f4(t[0].t4); d.hb_t78 = cvt_f4(t[0].t78);
+        assign_sets(extra, idx, nt, t, P.hb);
+        for (int k = 0; k < 16; ++k) P.hb.hb_eps[k] = k < nt * nt ? (float)t[k].eps : 0.f;
         break;
     }
-    case OXDNA_TERM_CROSS: {
-        auto* t = (const oxdna_crossstacking_params*)params;
-        require_uniform(t, nt, "cross-stacking", clear_cs);
-        DBaseBase& d = P.cs;
-        cvt3(d.base, t[0].base_site);
-        set_f2(d.f2, t[0].f2_k, t[0].f2_r0, t[0].f2_rc, t[0].f2_rlow, t[0].f2_rhigh, t[0].f2_rclow, t[0].f2_rchigh,
-               t[0].f2_blow, t[0].f2_bhigh);
-        d.cs_t1 = cvt_f4(t[0].t1); d.cs_t23 = cvt_f4(t[0].t23); d.cs_t4 = cvt_f4(t[0].t4); d.cs_t78 = cvt_f4(t[0].t78);
-        d.cs_t4_enabled = t[0].t4_enabled;
+    case OXDNA_TERM_CROSS:
+        assign_sets(extra, idx, nt, (const oxdna_crossstacking_params*)params, P.cs);
         break;
-    }
     case OXDNA_TERM_HBONDCROSS: {
         auto* t = (const oxdna_hbondcross_params*)params;
-        require_uniform(t, nt, "HB + cross-stacking", clear_hbx);
-        DBaseBase& d = P.hbx;
-        cvt3(d.base, t[0].base_site);
-        set_f1(d.f1, t[0].f1_a, t[0].f1_r0, t[0].f1_rlow, t[0].f1_rhigh, t[0].f1_rclow, t[0].f1_rchigh,
-               t[0].f1_blow, t[0].f1_bhigh);
-        d.shift_factor = (float)t[0].shift_factor;
-        for (int k = 0; k < 16; ++k) d.hb_eps[k] = k < nt * nt ? (float)t[k].eps : 0.f;
-        d.hb_t1 = cvt_f4(t[0].hb_t1); d.hb_t23 = cvt_f4(t[0].hb_t23);
-        d.hb_t4 = cvt_f4(t[0].hb_t4); d.hb_t78 = cvt_f4(t[0].hb_t78);
-        set_f2(d.f2, t[0].f2_k, t[0].f2_r0, t[0].f2_rc, t[0].f2_rlow, t[0].f2_rhigh, t[0].f2_rclow, t[0].f2_rchigh,
-               t[0].f2_blow, t[0].f2_bhigh);
-        d.cs_t1 = cvt_f4(t[0].cs_t1); d.cs_t23 = cvt_f4(t[0].cs_t23);
-        d.cs_t4 = cvt_f4(t[0].cs_t4); d.cs_t78 = cvt_f4(t[0].cs_t78);
-        d.cs_t4_enabled = t[0].cs_t4_enabled;
+        assign_sets(extra, idx, nt, t, P.hbx);
+        for (int k = 0; k < 16; ++k) P.hbx.hb_eps[k] = k < nt * nt ? (float)t[k].eps : 0.f;
         break;
     }
-    case OXDNA_TERM_COAX: {
-        auto* t = (const oxdna_coaxstacking_params*)params;
-        require_uniform(t, nt, "coaxial stacking", clear_cx);
-        DCoax& d = P.cx;
-        cvt3(d.stack, t[0].stack_site); cvt3(d.backref, t[0].stack_back_ref);
-        d.gamma = (float)t[0].gamma;
-        set_f2(d.f2, t[0].f2_k, t[0].f2_r0, t[0].f2_rc, t[0].f2_rlow, t[0].f2_rhigh, t[0].f2_rclow, t[0].f2_rchigh,
-               t[0].f2_blow, t[0].f2_bhigh);
-        d.t1 = cvt_f4(t[0].t1); d.t4 = cvt_f4(t[0].t4); d.t56 = cvt_f4(t[0].t56);
-        d.phi3 = cvt_f5(t[0].phi3); d.phi4 = cvt_f5(t[0].phi4);
-        d.t1_mode = t[0].t1_mode; d.phi3_enabled = t[0].phi3_enabled; d.rna_coax = t[0].rna_coax;
-        d.t1_sa = (float)t[0].t1_sa; d.t1_sb = (float)t[0].t1_sb;
-        if (d.t1_mode != 0 && d.t1_mode != 1) throw Fail("coaxial stacking: t1_mode must be 0 or 1");
-        // theta1's second piece: mode 0 f4(2 pi - theta1), nonzero for theta1 > 2 pi - t0 - tc
-        // (theta1 <= pi); mode 1 SA (theta1 - SB)^2, nonzero for theta1 > SB
-        const double t1_lo = d.t1_mode == 0 ? 2.0 * M_PI - t[0].t1[2] - t[0].t1[4] : t[0].t1_sb;  // (a, b, t0, ts, tc)
-        d.t1_cx = t1_lo >= M_PI ? -2.0f : (t1_lo <= 0.0 ? 2.0f : (float)std::cos(t1_lo));
+    case OXDNA_TERM_COAX:
+        assign_sets(extra, idx, nt, (const oxdna_coaxstacking_params*)params, P.cx);
         break;
-    }
-    case OXDNA_TERM_DEBYE: {
-        auto* t = (const oxdna_debye_params*)params;
-        require_uniform(t, nt, "Debye-Hueckel", keep_dh);
-        DDebye& d = P.dh;
-        cvt3(d.back, t[0].back_site);
-        d.minus_kappa = (float)t[0].minus_kappa; d.prefactor = (float)t[0].prefactor;
-        d.b_smooth = (float)t[0].b_smooth; d.rc = (float)t[0].rc; d.rhigh = (float)t[0].rhigh;
+    case OXDNA_TERM_DEBYE:
+        assign_sets(extra, idx, nt, (const oxdna_debye_params*)params, P.dh);
         break;
-    }
     default:
         throw Fail("unknown nonbonded term id");
     }
+}
+
+// Largest extras block (every term's further distinct sets together): with the
+// kernels' other LDS a workgroup stays well within the CU's LDS.
+constexpr size_t kMaxExtraBytes = 16 * 1024;
+
+// The parameter block the kernels read: DParams with the extras block right after it
+// (16-byte words). Sets P.poff (every type pair's set offsets) and P.xwords from the
+// terms' extras and set indices idx[term][tp].
+std::vector<uint4> param_blob(DParams& P, const std::vector<unsigned char>* extra, const uint8_t (*idx)[16]) {
+    static const size_t base_off[6] = {offsetof(DParams, ev), offsetof(DParams, hb), offsetof(DParams, cs),
+                                       offsetof(DParams, cx), offsetof(DParams, dh), offsetof(DParams, hbx)};
+    static const size_t set_size[6] = {sizeof(DExcVol), sizeof(DBaseBase), sizeof(DBaseBase),
+                                       sizeof(DCoax), sizeof(DDebye), sizeof(DBaseBase)};
+    size_t xoff[6], total = 0;
+    for (int t = 0; t < 6; ++t) {
+        xoff[t] = total;
+        total += extra[t].size();
+    }
+    if (total > kMaxExtraBytes)
+        throw Fail("per-type-pair parameter sets: " + std::to_string(total) + " bytes of distinct sets beyond the "
+                   "first of each term; at most " + std::to_string(kMaxExtraBytes));
+    for (int k = 0; k < 16; ++k) {
+        for (int t = 0; t < 6; ++t) {
+            const size_t j = idx[t][k];
+            P.poff[k][t] = (uint16_t)(j == 0 ? base_off[t] : sizeof(DParams) + xoff[t] + (j - 1) * set_size[t]);
+        }
+        P.poff[k][6] = P.poff[k][7] = 0;
+    }
+    P.xwords = (uint32_t)((total + 15) / 16);
+    std::vector<uint4> blob(sizeof(DParams) / 16 + P.xwords);
+    std::memcpy(blob.data(), &P, sizeof(DParams));
+    unsigned char* x = reinterpret_cast<unsigned char*>(blob.data() + sizeof(DParams) / 16);
+    std::memset(x, 0, (size_t)P.xwords * 16);
+    for (int t = 0; t < 6; ++t)
+        if (!extra[t].empty()) std::memcpy(x + xoff[t], extra[t].data(), extra[t].size());
+    return blob;
 }
 
 void load_bond_params(DBonded& d, const oxdna_bonded_params& s) {
@@ -1252,8 +1328,10 @@ uint32_t effective_mask(const oxdna_ctx* c, uint32_t mask) {
 
 void upload_params(oxdna_ctx* c) {
     if (!c->params_dirty) return;
-    c->dparams.alloc(1);
-    HIP_OK(hipMemcpyAsync(c->dparams.p, &c->P, sizeof(DParams), hipMemcpyHostToDevice, c->stream));
+    const std::vector<uint4> blob = param_blob(c->P, c->xset, c->pidx);
+    c->dparams.alloc(blob.size());
+    HIP_OK(hipMemcpyAsync(c->dparams.p, blob.data(), sizeof(uint4) * blob.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));  // the host blob goes out of scope
     c->params_dirty = false;
 }
 
@@ -1261,8 +1339,25 @@ void upload_params(oxdna_ctx* c) {
 // One templated launcher for every instantiation: TPP lanes per particle (8, 16, 32).
 void check_force_args(const ForceArgs& a);
 template<bool E, bool V, bool CNT, bool FUSED = false, bool FAR2 = false>
-void launch_k(int tpp, dim3 g, dim3 b, size_t lds, hipStream_t st, const ForceArgs& a) {
+void launch_k(int tpp, dim3 g, dim3 b, size_t rows_lds, hipStream_t st, const ForceArgs& a) {
     check_force_args(a);
+    if (a.xwords) {
+        // some table differs between type pairs: the general instantiation (8 or 32 lanes
+        // per particle, no fused integrator, one far entry per thread); its dynamic LDS
+        // starts with the parameter block
+        const size_t lds = sizeof(DParams) + 16 * (size_t)a.xwords + rows_lds;
+        if constexpr (FUSED || FAR2) {
+            throw Fail("internal: per-type-pair parameter sets on a fused or two-far-entry launch");
+        } else if (tpp == 8) {
+            hipLaunchKernelGGL((k_forces<32, E, V, CNT, false, false, true>), g, b, lds, st, a);
+        } else if (tpp == 32) {
+            hipLaunchKernelGGL((k_forces<8, E, V, CNT, false, false, true>), g, b, lds, st, a);
+        } else {
+            throw Fail("internal: per-type-pair parameter sets need 8 or 32 lanes per particle");
+        }
+        return;
+    }
+    const size_t lds = rows_lds;  // the parameter block is static LDS here
     // tpp lanes per particle: 256 / tpp particles per workgroup
     switch (tpp) {
     case 4: hipLaunchKernelGGL((k_forces<64, E, V, CNT, FUSED, FAR2>), g, b, lds, st, a); break;
@@ -1390,6 +1485,11 @@ void ensure_ready(oxdna_ctx* c) {
         c->nlist_valid = false;
     }
     upload_params(c);
+    // per-type-pair sets run on the general kernel instantiations: 8 or 32 lanes
+    if (c->P.xwords && c->tpp != 8 && c->tpp != 32) {
+        c->tpp = lds_bytes(c, 8) > kMaxLds ? 32 : 8;
+        c->drop_graph();
+    }
 }
 
 StepArgs step_args(oxdna_ctx* c, bool with_cells, uint64_t t);
@@ -1413,9 +1513,10 @@ ForceArgs force_args(oxdna_ctx* c, uint32_t mask, int accumulate, int rebuild, u
     a.ncu = c->ncu;
     a.bpart = c->bpart.p;
     a.binfo = c->binfo.p;
-    a.P = c->dparams.p;
+    a.P = reinterpret_cast<const DParams*>(c->dparams.p);
     for (int k = 0; k < 3; ++k) { a.L[k] = c->L[k]; a.Linv[k] = c->Linv[k]; }
     a.mask = mask;
+    a.xwords = c->P.xwords;
     a.force = c->force.p;
     a.torque = c->torque.p;
     a.virial = c->virial.p;
@@ -1571,6 +1672,7 @@ StepArgs step_args(oxdna_ctx* c, bool with_cells, uint64_t t) {
 // SIMD: a launch saved is time saved); on larger systems its one-active-lane-per-
 // particle epilogue would cost more issue slots than a kernel with a thread per particle.
 bool fuse_second_half(const oxdna_ctx* c) {
+    if (c->P.xwords) return false;  // per-type-pair sets: the general kernel, no fused integrator
     const size_t waves = ((size_t)c->N * c->tpp + 63) / 64;
     return waves <= 2048;  // resident at once: 2 waves per SIMD (the fused kernel's occupancy) on 256 CUs
 }
@@ -1623,6 +1725,16 @@ void build_graph(oxdna_ctx* c, uint32_t mask) {
 // Autotuner: lanes per particle (8, 16, 32) of the force kernel, timed at the
 // current state (rows current, no rebuild) with HIP events; keeps the fastest.
 void autotune(oxdna_ctx* c, uint32_t m) {
+    if (c->P.xwords) {
+        // per-type-pair sets: the general kernel runs 8 lanes per particle (32 when the rows
+        // are too long for that many particles per workgroup)
+        const int want = lds_bytes(c, 8) > kMaxLds ? 32 : 8;
+        if (c->tpp != want) {
+            c->tpp = want;
+            c->drop_graph();
+        }
+        return;
+    }
     if (c->fixed_tpp) {
         // the pinned value, unless the rows grew too long for its LDS (then 32 lanes)
         const int want = lds_bytes(c, c->fixed_tpp) > kMaxLds ? 32 : c->fixed_tpp;
@@ -1789,7 +1901,7 @@ EnsArgs ens_args(oxdna_ctx* c, uint32_t mask, uint32_t nsteps, uint32_t nevals, 
     a.bd = c->bd.p; a.bd_tq = c->bd_tq.p;
     a.charge = c->charge.p;
     a.bpart = c->bpart.p; a.binfo = c->binfo.p;
-    a.P = c->dparams.p;
+    a.P = reinterpret_cast<const DParams*>(c->dparams.p);
     a.mask = mask;
     a.force = c->force.p; a.torque = c->torque.p;
     a.rep_energy = c->rep_energy.p;
@@ -1835,12 +1947,19 @@ void launch_ens_nt(int nt, dim3 g, hipStream_t st, const EnsArgs& a) {
     throw Fail("internal: unsupported replica workgroup size");
 }
 
-// Launches the replica kernel with workgroup size nt (count: the counting variant).
+// Launches the replica kernel with workgroup size nt (count: the counting variant). With
+// per-type-pair parameter sets the general instantiation (512 particles, 512 threads)
+// runs every replica.
 void launch_ens(oxdna_ctx* c, int nt, bool count, const EnsArgs& a) {
     const int nmax = ens_nmax(c);
     if (nt < nmax || a.n_rep == 0 || !a.offsets || !a.order || !a.P || c->rep_nmax > (uint32_t)nmax)
         throw Fail("internal: replica kernel launch does not fit the replicas");
     dim3 g(a.n_rep);
+    if (c->P.xwords) {
+        if (count) hipLaunchKernelGGL((k_ensemble<512, 512, true, true>), g, dim3(512), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_ensemble<512, 512, false, true>), g, dim3(512), 0, c->stream, a);
+        return;
+    }
     if (count) {
         hipLaunchKernelGGL((k_ensemble<512, 512, true>), g, dim3(512), 0, c->stream, a);
         return;
@@ -1860,6 +1979,11 @@ int ens_default_nt(const oxdna_ctx* c) { return std::max(256, ens_nmax(c)); }
 // 512 threads, every power of two), timed as forces-only launches (the state is not
 // changed) with HIP events; keeps the fastest.
 void autotune_ens(oxdna_ctx* c, uint32_t m) {
+    if (c->P.xwords) {  // the general instantiation has one workgroup size
+        c->ens_nt = 512;
+        c->ens_tuned_mask = m;
+        return;
+    }
     if (c->ens_tuned_mask == m && c->ens_nt != 0) return;
     float best = 1e30f;
     int best_nt = ens_default_nt(c);
@@ -1888,7 +2012,7 @@ void autotune_ens(oxdna_ctx* c, uint32_t m) {
 void compute_ens(oxdna_ctx* c, uint32_t mask) {
     uint32_t m = effective_mask(c, mask);
     ensure_ready_ens(c, m);
-    int nt = c->ens_nt ? c->ens_nt : ens_default_nt(c);
+    int nt = c->P.xwords ? 512 : (c->ens_nt ? c->ens_nt : ens_default_nt(c));
     launch_ens(c, nt, false, ens_args(c, m, 0, 1, 1));
     hipLaunchKernelGGL(k_rep_sum, dim3(1), dim3(256), 0, c->stream, c->n_rep, c->rep_energy.p, c->energy.p);
     HIP_OK(hipMemsetAsync(c->virial.p, 0, sizeof(float) * 6 * (size_t)c->N, c->stream));
@@ -2101,14 +2225,28 @@ int oxdna_set_pair_params(oxdna_ctx* c, int term, int ntypes, const void* params
         if (!params) {
             c->pair_set[term] = false;
             c->rcut_max[term] = 0.0;
+            c->xset[term].clear();
+            for (int k = 0; k < 16; ++k) c->pidx[term][k] = 0;
         } else {
-            if (ntypes < 1 || ntypes > 4) throw Fail("oxdna_set_pair_params: ntypes must be 1..4");
+            if (ntypes < 1 || ntypes > 4)
+                throw Fail("oxdna_set_pair_params: ntypes must be 1..4 (the oxDNA type set A, G, C, T/U)");
             if (!rcut) throw Fail("oxdna_set_pair_params: rcut table is NULL");
-            load_pair_params(c->P, term, ntypes, params);
+            for (int k = 0; k < ntypes * ntypes; ++k)
+                if (!(rcut[k] >= 0.0)) throw Fail("oxdna_set_pair_params: negative or NaN cut-off");
+            DParams P = c->P;
+            std::vector<unsigned char> xset[6];
+            uint8_t pidx[6][16];
+            std::memcpy(pidx, c->pidx, sizeof(pidx));
+            for (int t = 0; t < 6; ++t)
+                if (t != term) xset[t] = c->xset[t];
+            load_pair_params(P, xset[term], pidx[term], term, ntypes, params);
+            (void)param_blob(P, xset, pidx);  // checks the size of the extras block
+            c->P = P;
+            c->xset[term] = xset[term];
+            std::memcpy(c->pidx, pidx, sizeof(pidx));
             double rmax = 0.0;
             for (int k = 0; k < 16; ++k) {
                 double r = k < ntypes * ntypes ? rcut[k] : 0.0;
-                if (r < 0) throw Fail("oxdna_set_pair_params: negative cut-off");
                 c->P.rcutsq[term][k] = (float)(r * r);
                 rmax = std::max(rmax, r);
             }
@@ -2517,12 +2655,14 @@ namespace {
 // demand, kept until the process ends). Calls that overlap on one device race on it.
 struct DriverScratch {
     DBuf<float4> fr0, fr1, fr2;
-    DBuf<DParams> P;
+    DBuf<uint4> P;  // parameter block: DParams, then its extras
     DBuf<double> epart;
     DBuf<int> bpart, binfo;
     DBuf<uint32_t> err, zeros;
     DBuf<size_t> zhead;
-    std::vector<DParams> staged = std::vector<DParams>(8);
+    // host copies of the last parameter blocks: an asynchronous copy reads its source
+    // after the call returns
+    std::vector<std::vector<uint4>> staged = std::vector<std::vector<uint4>>(8);
     int ring = 0;
 };
 DriverScratch g_scratch[64];
@@ -2533,13 +2673,13 @@ DriverScratch& scratch_for(int& dev) {
     return g_scratch[dev];
 }
 
-const DParams* stage_params(DriverScratch& s, const DParams& P, hipStream_t st) {
-    s.P.alloc(1);
-    DParams& h = s.staged[s.ring];
+const DParams* stage_params(DriverScratch& s, std::vector<uint4> blob, hipStream_t st) {
+    s.P.alloc(blob.size());
+    std::vector<uint4>& h = s.staged[s.ring];
     s.ring = (s.ring + 1) % (int)s.staged.size();
-    h = P;
-    HIP_OK(hipMemcpyAsync(s.P.p, &h, sizeof(DParams), hipMemcpyHostToDevice, st));
-    return s.P.p;
+    h = std::move(blob);
+    HIP_OK(hipMemcpyAsync(s.P.p, h.data(), sizeof(uint4) * h.size(), hipMemcpyHostToDevice, st));
+    return reinterpret_cast<const DParams*>(s.P.p);
 }
 
 void driver_frames(DriverScratch& s, uint32_t N, const double* d_pos, const double* d_orient, const double* d_charge,
@@ -2552,39 +2692,45 @@ void driver_frames(DriverScratch& s, uint32_t N, const double* d_pos, const doub
 // Lanes per particle of the stateless drivers' force launches.
 constexpr int kDriverTpp = 8;
 
-// Checks the arguments of a stateless pair-driver call and builds its parameter block.
-DParams pair_driver_params(const char* who, int term, unsigned int N, float* d_force, float* d_torque,
-                           const double* d_pos, const double* d_orientation, const double* d_charge,
-                           const oxdna_box* box, const unsigned int* d_n_neigh, const unsigned int* d_nlist,
-                           const size_t* d_head_list, int ntypes, const void* params, const double* rcutsq) {
+// Checks the arguments of a stateless pair-driver call and builds its parameter block
+// (DParams, then the extras block of its per-type-pair sets).
+std::vector<uint4> pair_driver_params(const char* who, int term, unsigned int N, float* d_force, float* d_torque,
+                                      const double* d_pos, const double* d_orientation, const double* d_charge,
+                                      const oxdna_box* box, const unsigned int* d_n_neigh, const unsigned int* d_nlist,
+                                      const size_t* d_head_list, int ntypes, const void* params,
+                                      const double* rcutsq) {
     const std::string w(who);
     if (term < 0 || term >= OXDNA_TERM_BONDED) throw Fail(w + ": not a nonbonded term id");
-    if (ntypes < 1 || ntypes > 4) throw Fail(w + ": ntypes must be 1..4");
+    if (ntypes < 1 || ntypes > 4) throw Fail(w + ": ntypes must be 1..4 (the oxDNA type set A, G, C, T/U)");
     if (!params || !rcutsq || !box) throw Fail(w + ": NULL parameter table, cut-offs or box");
     DParams P{};
-    load_pair_params(P, term, ntypes, params);
+    std::vector<unsigned char> xset[6];
+    uint8_t pidx[6][16] = {};
+    load_pair_params(P, xset[term], pidx[term], term, ntypes, params);
     for (int k = 0; k < ntypes * ntypes; ++k) {
         if (!(rcutsq[k] >= 0.0)) throw Fail(w + ": negative or NaN squared cut-off");
         P.rcutsq[term][k] = (float)rcutsq[k];
     }
     P.ntypes = ntypes;
     P.mask = 1u << term;
-    if (N == 0) return P;
+    std::vector<uint4> blob = param_blob(P, xset, pidx);
+    if (N == 0) return blob;
     if (!d_force || !d_torque || !d_pos || !d_orientation || !d_n_neigh || !d_nlist || !d_head_list)
         throw Fail(w + ": NULL device array");
     if (term == OXDNA_TERM_DEBYE && !d_charge) throw Fail(w + ": Debye-Hueckel needs charges");
     if (N > kIndexMask) throw Fail(w + ": N must be below 2^30 (neighbour indices keep 30 bits)");
-    return P;
+    return blob;
 }
 
 // Frames, the force launch over the caller's rows and the energy reduction of one
 // stateless pair-driver call, on the given device workspace (dP: the parameter block
-// on the device; fr0..fr2: N float4 each; epart: 8 doubles per force workgroup).
+// on the device, xwords its extras; fr0..fr2: N float4 each; epart: 8 doubles per force
+// workgroup).
 void pair_driver_launch(int term, float* d_force, float* d_torque, float* d_virial, size_t virial_pitch,
                         unsigned int N, const double* d_pos, const double* d_orientation, const double* d_charge,
                         const oxdna_box* box, const unsigned int* d_n_neigh, const unsigned int* d_nlist,
-                        const size_t* d_head_list, const DParams* dP, float4* fr0, float4* fr1, float4* fr2,
-                        double* epart, double* d_term_energy, hipStream_t st) {
+                        const size_t* d_head_list, const DParams* dP, uint32_t xwords, float4* fr0, float4* fr1,
+                        float4* fr2, double* epart, double* d_term_energy, hipStream_t st) {
     hipLaunchKernelGGL(k_frames, dim3(blocks_for(N)), dim3(kBlock), 0, st, N, (const double4*)d_orientation, d_charge,
                        (const double4*)d_pos, fr0, fr1, fr2);
     const uint32_t blocks = blocks_for((size_t)N * kDriverTpp);
@@ -2599,6 +2745,7 @@ void pair_driver_launch(int term, float* d_force, float* d_torque, float* d_viri
     a.P = dP;
     for (int k = 0; k < 3; ++k) { a.L[k] = box->L[k]; a.Linv[k] = 1.0 / box->L[k]; }
     a.mask = 1u << term;
+    a.xwords = xwords;
     a.force = (float4*)d_force; a.torque = (float4*)d_torque;
     a.virial = d_virial; a.vpitch = virial_pitch;
     a.epart = epart;
@@ -2617,7 +2764,7 @@ void pair_driver_launch(int term, float* d_force, float* d_torque, float* d_viri
 constexpr uint32_t kNoAllocMaxN = 65536;
 constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 constexpr size_t kNaParams = 0;
-constexpr size_t kNaFrames = align256(sizeof(DParams));
+constexpr size_t kNaFrames = align256(sizeof(DParams) + kMaxExtraBytes);
 constexpr size_t kNaEpart = kNaFrames + align256(3 * (size_t)kNoAllocMaxN * sizeof(float4));
 constexpr size_t kNaBytes =
     kNaEpart + align256(((size_t)kNoAllocMaxN * kDriverTpp + kBlock - 1) / kBlock * 8 * sizeof(double));
@@ -2645,26 +2792,26 @@ unsigned char* noalloc_scratch() {
     return g_scratch_ptr[dev];
 }
 
-// The parameter block goes to the device as kernel arguments (two 2 KiB halves), so a
-// call makes no host-to-device copy from pageable memory.
-struct ParamHalf {
+// The parameter block goes to the device as kernel arguments (2 KiB per launch: two
+// for DParams, up to eight more for per-type-pair sets), so a call makes no host-to-
+// device copy from pageable memory.
+struct ParamChunk {
     uint4 w[128];
 };
-static_assert(sizeof(DParams) <= 2 * sizeof(ParamHalf), "the parameter block travels in two halves");
+static_assert(sizeof(DParams) % sizeof(uint4) == 0, "DParams is a whole number of 16-byte words");
 
-__global__ void k_put_params(ParamHalf h, uint4* dst, uint32_t nwords) {
+__global__ void k_put_params(ParamChunk h, uint4* dst, uint32_t nwords) {
     for (uint32_t k = threadIdx.x; k < nwords; k += blockDim.x) dst[k] = h.w[k];
 }
 
-void put_params(const DParams& P, DParams* dP, hipStream_t st) {
-    constexpr uint32_t words = (uint32_t)(sizeof(DParams) / sizeof(uint4));
-    static_assert(sizeof(DParams) % sizeof(uint4) == 0, "DParams is a whole number of 16-byte words");
-    const uint4* src = (const uint4*)&P;
+void put_params(const std::vector<uint4>& blob, uint4* dst, hipStream_t st) {
+    if (blob.size() * sizeof(uint4) > sizeof(DParams) + kMaxExtraBytes) throw Fail("internal: parameter block too large");
+    const uint32_t words = (uint32_t)blob.size();
     for (uint32_t h0 = 0; h0 < words; h0 += 128) {
-        ParamHalf h{};
+        ParamChunk h{};
         const uint32_t n = std::min<uint32_t>(128u, words - h0);
-        std::memcpy(h.w, src + h0, n * sizeof(uint4));
-        hipLaunchKernelGGL(k_put_params, dim3(1), dim3(128), 0, st, h, (uint4*)dP + h0, n);
+        std::memcpy(h.w, blob.data() + h0, n * sizeof(uint4));
+        hipLaunchKernelGGL(k_put_params, dim3(1), dim3(128), 0, st, h, dst + h0, n);
     }
 }
 
@@ -2695,18 +2842,20 @@ int oxdna_pair_forces(int term, float* d_force, float* d_torque, float* d_virial
                       const size_t* d_head_list, int ntypes, const void* params, const double* rcutsq,
                       double* d_term_energy, void* stream) {
     return guard(nullptr, [&] {
-        const DParams P = pair_driver_params("oxdna_pair_forces", term, N, d_force, d_torque, d_pos, d_orientation,
-                                             d_charge, box, d_n_neigh, d_nlist, d_head_list, ntypes, params, rcutsq);
+        std::vector<uint4> blob = pair_driver_params("oxdna_pair_forces", term, N, d_force, d_torque, d_pos,
+                                                     d_orientation, d_charge, box, d_n_neigh, d_nlist, d_head_list,
+                                                     ntypes, params, rcutsq);
         if (N == 0) return;
+        const uint32_t xwords = reinterpret_cast<const DParams*>(blob.data())->xwords;
         hipStream_t st = (hipStream_t)stream;
         int dev;
         DriverScratch& s = scratch_for(dev);
-        const DParams* dP = stage_params(s, P, st);
+        const DParams* dP = stage_params(s, std::move(blob), st);
         s.fr0.alloc(N); s.fr1.alloc(N); s.fr2.alloc(N);
         s.epart.alloc((size_t)blocks_for((size_t)N * kDriverTpp) * 8);
         pair_driver_launch(term, d_force, d_torque, d_virial, virial_pitch, N, d_pos, d_orientation, d_charge, box,
-                           d_n_neigh, d_nlist, d_head_list, dP, s.fr0.p, s.fr1.p, s.fr2.p, s.epart.p, d_term_energy,
-                           st);
+                           d_n_neigh, d_nlist, d_head_list, dP, xwords, s.fr0.p, s.fr1.p, s.fr2.p, s.epart.p,
+                           d_term_energy, st);
     });
 }
 
@@ -2716,22 +2865,23 @@ int oxdna_pair_forces_noalloc(int term, float* d_force, float* d_torque, float* 
                               const unsigned int* d_nlist, const size_t* d_head_list, int ntypes,
                               const void* params, const double* rcutsq, double* d_term_energy, void* stream) {
     return guard(nullptr, [&] {
-        const DParams P = pair_driver_params("oxdna_pair_forces_noalloc", term, N, d_force, d_torque, d_pos,
-                                             d_orientation, d_charge, box, d_n_neigh, d_nlist, d_head_list, ntypes,
-                                             params, rcutsq);
+        const std::vector<uint4> blob =
+            pair_driver_params("oxdna_pair_forces_noalloc", term, N, d_force, d_torque, d_pos, d_orientation, d_charge,
+                               box, d_n_neigh, d_nlist, d_head_list, ntypes, params, rcutsq);
         if (N > kNoAllocMaxN)
             throw Fail("oxdna_pair_forces_noalloc: " + std::to_string(N) + " particles; the fixed scratch holds at most " +
                        std::to_string(kNoAllocMaxN) + " (use oxdna_pair_forces)");
         unsigned char* base = noalloc_scratch();  // resolves the array's address on this device once
         if (N == 0) return;
         hipStream_t st = (hipStream_t)stream;
-        DParams* dP = (DParams*)(base + kNaParams);
-        put_params(P, dP, st);
+        uint4* dP = (uint4*)(base + kNaParams);
+        put_params(blob, dP, st);
         float4* fr = (float4*)(base + kNaFrames);
         double* epart = (double*)(base + kNaEpart);
         pair_driver_launch(term, d_force, d_torque, d_virial, virial_pitch, N, d_pos, d_orientation, d_charge, box,
-                           d_n_neigh, d_nlist, d_head_list, dP, fr, fr + kNoAllocMaxN, fr + 2 * (size_t)kNoAllocMaxN,
-                           epart, d_term_energy, st);
+                           d_n_neigh, d_nlist, d_head_list, (const DParams*)dP,
+                           reinterpret_cast<const DParams*>(blob.data())->xwords, fr, fr + kNoAllocMaxN,
+                           fr + 2 * (size_t)kNoAllocMaxN, epart, d_term_energy, st);
     });
 }
 
@@ -2756,7 +2906,9 @@ int oxdna_bonded_forces(float* d_force, float* d_torque, float* d_virial, size_t
         P.n_bond_types = (int)n_bond_types;
         P.ntypes = 4;
         P.mask = 1u << OXDNA_TERM_BONDED;
-        const DParams* dP = stage_params(s, P, st);
+        std::vector<uint4> blob(sizeof(DParams) / sizeof(uint4));
+        std::memcpy(blob.data(), &P, sizeof(DParams));
+        const DParams* dP = stage_params(s, std::move(blob), st);
         driver_frames(s, N, d_pos, d_orientation, nullptr, st);
         s.bpart.alloc(2 * (size_t)N); s.binfo.alloc(2 * (size_t)N); s.err.alloc(1);
         HIP_OK(hipMemsetAsync(s.err.p, 0, sizeof(uint32_t), st));

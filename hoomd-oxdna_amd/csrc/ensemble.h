@@ -75,7 +75,9 @@ struct EnsLds {
     uint32_t n_near, n_far;
 };
 
-template<int NMAX, int NT, bool COUNT>
+// MULTI: some table differs between type pairs (each pair finds its sets through their
+// offsets in the parameter block; otherwise the base sets, at fixed offsets).
+template<int NMAX, int NT, bool COUNT, bool MULTI = false>
 __global__ __launch_bounds__(NT) void k_ensemble(EnsArgs a) {
     static_assert(NT >= NMAX && NT % 64 == 0, "one thread per particle at least");
     __shared__ EnsLds<NMAX, NT> S;
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(NT) void k_ensemble(EnsArgs a) {
                     const V Rc = -dr;
                     V Fp = zero3(), Tp = zero3();
                     const uint32_t m = e < nn ? mask : (mask & (1u << T_DEBYE));
-                    const float ep = pair_terms<COUNT>(P, m, tp, rsq, Rc, fi, fj, g0.w * h0.w, Fp, Tp, et, cnt);
+                    const float ep = pair_terms<COUNT, MULTI>(P, m, tp, rsq, Rc, fi, fj, g0.w * h0.w, Fp, Tp, et, cnt);
                     if (Fp.x != 0.f || Fp.y != 0.f || Fp.z != 0.f || Tp.x != 0.f || Tp.y != 0.f || Tp.z != 0.f) {
                         const V Tj = cross(Rc, Fp) - Tp;
                         atomicAdd(&S.F[0][i], Fp.x); atomicAdd(&S.F[1][i], Fp.y); atomicAdd(&S.F[2][i], Fp.z);

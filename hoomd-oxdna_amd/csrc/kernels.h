@@ -4,6 +4,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "integrator.h"
@@ -63,24 +64,39 @@ __device__ __forceinline__ uint32_t cell_index(double x, double y, double z, con
 // Every selected nonbonded term of one pair, i half: adds the force on i to Fp and
 // the torque on i to T, the per-term energies to et (split_energy slots) and returns
 // the pair energy. Rc = r_j - r_i (minimum image), rsq = |Rc|^2, tp the type-pair
-// index. With COUNT, cnt[2 t] counts evaluations of term t inside its cut-off and
-// cnt[2 t + 1] those with a nonzero energy.
-template<bool COUNT>
+// index, P the start of the parameter block. MULTI: some table differs between type
+// pairs, so each term's set is found through P.poff (else the base sets, at offsets
+// fixed at compile time). With COUNT, cnt[2 t] counts evaluations of term t inside its
+// cut-off and cnt[2 t + 1] those with a nonzero energy.
+template<bool COUNT, bool MULTI>
 __device__ __forceinline__ float pair_terms(const DParams& P, uint32_t mask, int tp, float rsq, V Rc,
                                             const Frame& fi, const Frame& fj, float qiqj, V& Fp, V& T,
                                             float et[8], uint32_t cnt[14]) {
+    // byte offsets of this type pair's sets
+    uint4 po = make_uint4(0u, 0u, 0u, 0u);
+    if constexpr (MULTI) po = *reinterpret_cast<const uint4*>(&P.poff[tp][0]);
+    const unsigned char* pb = reinterpret_cast<const unsigned char*>(&P);
+    auto at = [&](int t, uint32_t base_off) -> const unsigned char* {
+        if constexpr (MULTI) {
+            const uint32_t w = t < 2 ? po.x : (t < 4 ? po.y : po.z);
+            return pb + ((t & 1) ? w >> 16 : w & 0xffffu);
+        } else {
+            return pb + base_off;
+        }
+    };
     float ep = 0.f;
     if ((mask & (1u << T_EXCVOL)) && rsq < P.rcutsq[T_EXCVOL][tp]) {
-        float e = ev_pair(Rc, fi, fj, P.ev, Fp, T);
+        float e = ev_pair(Rc, fi, fj, *reinterpret_cast<const DExcVol*>(at(T_EXCVOL, offsetof(DParams, ev))), Fp, T);
         et[S_NEXC] += e; ep += e;
         if (COUNT) { cnt[2 * T_EXCVOL]++; cnt[2 * T_EXCVOL + 1] += e != 0.f; }
     }
     if ((mask & (1u << T_HBOND)) && rsq < P.rcutsq[T_HBOND][tp]) {
+        const DBaseBase& S = *reinterpret_cast<const DBaseBase*>(at(T_HBOND, offsetof(DParams, hb)));
         BB g;
-        if (bb_geom(Rc, fi, fj, P.hb.base, g)) {
+        if (bb_geom(Rc, fi, fj, S.base, g)) {
             float eps = P.hb.hb_eps[tp], rv, rd;
-            f1_vd(g.rmod, P.hb.f1, eps, eps * P.hb.shift_factor, rv, rd);
-            float e = bb_term(g, fi, fj, rv, rd, P.hb.hb_t1, P.hb.hb_t23, P.hb.hb_t4, P.hb.hb_t78,
+            f1_vd(g.rmod, S.f1, eps, eps * S.shift_factor, rv, rd);
+            float e = bb_term(g, fi, fj, rv, rd, S.hb_t1, S.hb_t23, S.hb_t4, S.hb_t78,
                               false, true, Fp, T);
             et[S_HB] += e; ep += e;
             if (COUNT) cnt[2 * T_HBOND + 1] += e != 0.f;
@@ -88,27 +104,29 @@ __device__ __forceinline__ float pair_terms(const DParams& P, uint32_t mask, int
         if (COUNT) cnt[2 * T_HBOND]++;
     }
     if ((mask & (1u << T_CROSS)) && rsq < P.rcutsq[T_CROSS][tp]) {
+        const DBaseBase& S = *reinterpret_cast<const DBaseBase*>(at(T_CROSS, offsetof(DParams, cs)));
         BB g;
-        if (bb_geom(Rc, fi, fj, P.cs.base, g)) {
+        if (bb_geom(Rc, fi, fj, S.base, g)) {
             float rv, rd;
-            f2_vd(g.rmod, P.cs.f2, rv, rd);
-            float e = bb_term(g, fi, fj, rv, rd, P.cs.cs_t1, P.cs.cs_t23, P.cs.cs_t4, P.cs.cs_t78,
-                              true, P.cs.cs_t4_enabled != 0, Fp, T);
+            f2_vd(g.rmod, S.f2, rv, rd);
+            float e = bb_term(g, fi, fj, rv, rd, S.cs_t1, S.cs_t23, S.cs_t4, S.cs_t78,
+                              true, S.cs_t4_enabled != 0, Fp, T);
             et[S_CR] += e; ep += e;
             if (COUNT) cnt[2 * T_CROSS + 1] += e != 0.f;
         }
         if (COUNT) cnt[2 * T_CROSS]++;
     }
     if ((mask & (1u << T_HBONDCROSS)) && rsq < P.rcutsq[T_HBONDCROSS][tp]) {
+        const DBaseBase& S = *reinterpret_cast<const DBaseBase*>(at(T_HBONDCROSS, offsetof(DParams, hbx)));
         BB g;
-        if (bb_geom(Rc, fi, fj, P.hbx.base, g)) {
+        if (bb_geom(Rc, fi, fj, S.base, g)) {
             const float eps = P.hbx.hb_eps[tp];
             float rv1, rd1, rv2, rd2;
-            f1_vd(g.rmod, P.hbx.f1, eps, eps * P.hbx.shift_factor, rv1, rd1);
-            f2_vd(g.rmod, P.hbx.f2, rv2, rd2);
-            const bool t4cs = P.hbx.cs_t4_enabled != 0;
-            const bool on1 = bb_active(g, rv1, P.hbx.hb_t1, P.hbx.hb_t23, P.hbx.hb_t4, P.hbx.hb_t78, false, true);
-            const bool on2 = bb_active(g, rv2, P.hbx.cs_t1, P.hbx.cs_t23, P.hbx.cs_t4, P.hbx.cs_t78, true, t4cs);
+            f1_vd(g.rmod, S.f1, eps, eps * S.shift_factor, rv1, rd1);
+            f2_vd(g.rmod, S.f2, rv2, rd2);
+            const bool t4cs = S.cs_t4_enabled != 0;
+            const bool on1 = bb_active(g, rv1, S.hb_t1, S.hb_t23, S.hb_t4, S.hb_t78, false, true);
+            const bool on2 = bb_active(g, rv2, S.cs_t1, S.cs_t23, S.cs_t4, S.cs_t78, true, t4cs);
             float e1 = 0.f, e2 = 0.f;
             if (on1 || on2) {
                 BBAng an;  // one arccos per cosine for both terms
@@ -116,14 +134,12 @@ __device__ __forceinline__ float pair_terms(const DParams& P, uint32_t mask, int
                 BBW w;  // both terms' weights, assembled once
                 bbw_zero(w);
                 if (on1) {
-                    const BBW w1 = bb_weights(g, an, rv1, rd1, P.hbx.hb_t1, P.hbx.hb_t23, P.hbx.hb_t4, P.hbx.hb_t78,
-                                              false, true);
+                    const BBW w1 = bb_weights(g, an, rv1, rd1, S.hb_t1, S.hb_t23, S.hb_t4, S.hb_t78, false, true);
                     e1 = w1.e;
                     bbw_add(w, w1);
                 }
                 if (on2) {
-                    const BBW w2 = bb_weights(g, an, rv2, rd2, P.hbx.cs_t1, P.hbx.cs_t23, P.hbx.cs_t4, P.hbx.cs_t78,
-                                              true, t4cs);
+                    const BBW w2 = bb_weights(g, an, rv2, rd2, S.cs_t1, S.cs_t23, S.cs_t4, S.cs_t78, true, t4cs);
                     e2 = w2.e;
                     bbw_add(w, w2);
                 }
@@ -135,12 +151,12 @@ __device__ __forceinline__ float pair_terms(const DParams& P, uint32_t mask, int
         if (COUNT) cnt[2 * T_HBONDCROSS]++;
     }
     if ((mask & (1u << T_COAX)) && rsq < P.rcutsq[T_COAX][tp]) {
-        float e = coax_pair(Rc, fi, fj, P.cx, Fp, T);
+        float e = coax_pair(Rc, fi, fj, *reinterpret_cast<const DCoax*>(at(T_COAX, offsetof(DParams, cx))), Fp, T);
         et[S_CX] += e; ep += e;
         if (COUNT) { cnt[2 * T_COAX]++; cnt[2 * T_COAX + 1] += e != 0.f; }
     }
     if ((mask & (1u << T_DEBYE)) && rsq < P.rcutsq[T_DEBYE][tp]) {
-        float e = dh_pair(Rc, fi, fj, qiqj, P.dh, Fp, T);
+        float e = dh_pair(Rc, fi, fj, qiqj, *reinterpret_cast<const DDebye*>(at(T_DEBYE, offsetof(DParams, dh))), Fp, T);
         et[S_DH] += e; ep += e;
         if (COUNT) { cnt[2 * T_DEBYE]++; cnt[2 * T_DEBYE + 1] += e != 0.f; }
     }
@@ -270,6 +286,8 @@ struct ForceArgs {
     const DParams* P;
     double L[3], Linv[3];
     uint32_t mask;
+    uint32_t xwords;       // size of P's extras block in 16-byte words (in the padding after mask;
+                           // nonzero: some table differs between type pairs)
     float4* force;
     float4* torque;
     float* virial;

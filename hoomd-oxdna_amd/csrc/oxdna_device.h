@@ -95,8 +95,13 @@ struct DBonded {
 
 enum { MAX_BOND_TYPES = 4 };
 
-// All term parameters of one force field (uniform over type pairs apart from the
-// per-type-pair tables); read with scalar loads.
+// All term parameters of one force field: the parameter block is this struct followed
+// by an extras block (in global memory and, in the force kernel, at the start of its
+// dynamic LDS). Each nonbonded term has a base parameter set (the fields below) and,
+// when its table differs between type pairs, further distinct sets in the extras:
+// poff[tp][t] is the byte offset, from the start of the block, of type pair tp's set of
+// term t (the base set's own offset when the tables are uniform). The hydrogen-bonding
+// epsilon of every type pair stays in the base set's hb_eps table.
 struct alignas(16) DParams {  // a multiple of 16 bytes: kernels copy it to LDS in 16-byte words
     uint32_t mask;            // bits of oxdna_term present
     int ntypes;
@@ -109,6 +114,10 @@ struct alignas(16) DParams {  // a multiple of 16 bytes: kernels copy it to LDS 
     DDebye dh;
     int n_bond_types;
     DBonded bond[MAX_BOND_TYPES];
+    // last, so that the sets above keep their offsets (and the alignment their reads
+    // are merged on)
+    uint16_t poff[16][8];     // per type pair: byte offset of each nonbonded term's set (0-5)
+    uint32_t xwords;          // size of the extras block in 16-byte words (0: every table uniform)
 };
 
 // ------------------------------------------------------------------ building blocks

@@ -177,7 +177,9 @@ int oxdna_device_name(int device, char* buf, size_t buflen);
  * reference's per-term instantiation of HOOMD's gpu_compute_pair_aniso_forces<E>
  * (src/AnisoPotentialPairOxDNA{ExcVol,HBond,CrossStacking,HBondCross,CoaxStacking,
  * Debye}GPUKernel.cu:14-18) driven by AnisoPotentialPairGPU<E>::computeForces.
- *   params:  ntypes*ntypes table of the term's struct above (row-major [ti][tj])
+ *   params:  ntypes*ntypes table of the term's struct above (row-major [ti][tj]),
+ *            any field may differ between type pairs (each pair uses its own struct);
+ *            ntypes is 1..4 (the oxDNA type set A, G, C, T/U)
  *   rcutsq:  ntypes*ntypes table of squared COM cut-offs (host memory)
  *   d_charge may be NULL for every term except OXDNA_TERM_DEBYE.
  *   Outputs are overwritten: force.xyz on i, force.w = half the pair energies of
@@ -253,8 +255,13 @@ int oxdna_set_momenta(oxdna_ctx* ctx, const double* vel, const double* angmom);
 int oxdna_set_bonds(oxdna_ctx* ctx, unsigned int nb, const int32_t* ab, const int32_t* btype);
 
 /* Term parameters. term is one of oxdna_term except OXDNA_TERM_BONDED. The table is
- * ntypes*ntypes structs (row-major), rcut is ntypes*ntypes COM cut-offs. Passing
- * params = NULL removes the term. */
+ * ntypes*ntypes structs (row-major [ti][tj]), rcut is ntypes*ntypes COM cut-offs; any
+ * field may differ between type pairs (the reference's param_type per type pair), each
+ * type pair being evaluated with its own struct. Limits: ntypes is 1..4 (the oxDNA type
+ * set A, G, C, T/U, which the bonded st_eps[4][4] table also assumes), and the distinct
+ * structs beyond the first of every term together take at most 16 KiB (e.g. 40 distinct
+ * fused HB + cross-stacking structs); beyond either the call fails with an error.
+ * Passing params = NULL removes the term. */
 int oxdna_set_pair_params(oxdna_ctx* ctx, int term, int ntypes, const void* params,
                           const double* rcut);
 int oxdna_set_bond_params(oxdna_ctx* ctx, unsigned int n_bond_types,
