@@ -80,6 +80,11 @@ struct EnsLds {
 template<int NMAX, int NT, bool COUNT, bool MULTI = false>
 __global__ __launch_bounds__(NT) void k_ensemble(EnsArgs a) {
     static_assert(NT >= NMAX && NT % 64 == 0, "one thread per particle at least");
+    // A grid of exactly one workgroup per replica, checked against the grid size: reading
+    // the grid size also gives the kernel the hidden-argument block that every other
+    // kernel of the library has (without it the argument segment ends at EnsArgs, and a
+    // tool that reads a dispatch's hidden arguments reads past its end).
+    if (gridDim.x != a.n_rep) return;
     __shared__ EnsLds<NMAX, NT> S;
     const uint32_t t = threadIdx.x;
     const uint32_t r = a.order[blockIdx.x];
