@@ -156,9 +156,6 @@ def bench_batched(name, reps, forces, kT, steps, warmup, census, bias=None, samp
     achieved = flops / (avg_ms.value * 1e-3) / 1e12
     tpp, nt = C.c_int(), C.c_int()
     assert lib.oxdna_get_tuning(ctx, C.byref(tpp), C.byref(nt)) == 0
-    nmax = 64
-    while nmax < int(np.diff(ens.offsets).max()):
-        nmax *= 2
     # plain path, one replica at a time
     pick = np.linspace(0, n_rep - 1, min(sample, n_rep)).round().astype(int) if sample > 0 else []
     plain_steps, plain_time = 0, 0.0
@@ -186,7 +183,7 @@ def bench_batched(name, reps, forces, kT, steps, warmup, census, bias=None, samp
         "speedup_vs_plain": round(rate / plain_rate, 2) if plain_rate else None,
         "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 6), "traffic": None,
-                     "kernel": f"k_ensemble<{nmax}, {nt.value}> (forces only)", "kernel_ms": round(avg_ms.value, 5),
+                     "kernel": f"k_ensemble<{nt.value}, false, false> (forces only)", "kernel_ms": round(avg_ms.value, 5),
                      "flops_per_launch": flops},
         "energy_per_nt": {k: round(float(v) / int(sizes.sum()), 6)
                           for k, v in zip(ensemble.REPLICA_ENERGY_NAMES, e.sum(axis=0))},

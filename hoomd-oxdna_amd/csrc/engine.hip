@@ -983,6 +983,7 @@ constexpr uint32_t kMaxNeigh = 768;  // 8 particles x 2 rows x 768 x 4 B = 48 Ki
 struct oxdna_ctx {
     int device = 0;
     uint32_t ncu = 0;  // compute units of the device
+    uint32_t lds_block = 65536, lds_cu = 65536;  // LDS bytes per workgroup, per compute unit
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string err;
@@ -1913,6 +1914,8 @@ EnsArgs ens_args(oxdna_ctx* c, uint32_t mask, uint32_t nsteps, uint32_t nevals, 
     a.nevals = nevals;
     a.want_e = want_e;
     a.counts = nullptr;
+    a.ncap = 0;  // set by launch_ens
+    a.cap = 0;
     // near: inside any cut-off but Debye-Hueckel's; far: inside Debye-Hueckel's only
     float rn = 0.f, rf = 0.f;
     for (int t = 0; t < 6; ++t) {
@@ -1928,49 +1931,53 @@ EnsArgs ens_args(oxdna_ctx* c, uint32_t mask, uint32_t nsteps, uint32_t nevals, 
     return a;
 }
 
-// Smallest replica capacity (power of two, 64..512) that holds the largest replica.
+// Smallest replica workgroup (power of two, 64..512 threads) that holds the largest
+// replica: one thread per particle.
 int ens_nmax(const oxdna_ctx* c) {
     int nmax = 64;
     while ((uint32_t)nmax < c->rep_nmax) nmax *= 2;
     return nmax;
 }
 
-template<int NMAX>
-void launch_ens_nt(int nt, dim3 g, hipStream_t st, const EnsArgs& a) {
-    switch (nt) {
-    case 64: if constexpr (NMAX <= 64) { hipLaunchKernelGGL((k_ensemble<NMAX, 64, false>), g, dim3(64), 0, st, a); return; } break;
-    case 128: if constexpr (NMAX <= 128) { hipLaunchKernelGGL((k_ensemble<NMAX, 128, false>), g, dim3(128), 0, st, a); return; } break;
-    case 256: if constexpr (NMAX <= 256) { hipLaunchKernelGGL((k_ensemble<NMAX, 256, false>), g, dim3(256), 0, st, a); return; } break;
-    case 512: hipLaunchKernelGGL((k_ensemble<NMAX, 512, false>), g, dim3(512), 0, st, a); return;
-    default: break;
+// LDS shape of a replica launch on nt-thread workgroups: particle capacity (the largest
+// replica, rounded up to 8) and pair records per pass, as many as fit the LDS share of
+// one workgroup when ceil(replicas / compute units) workgroups share a compute unit (at
+// most 8 wavefronts per compute unit: the kernel's registers allow 2 per SIMD). At least
+// ncap + 64 records, so a pass holds any particle's entries and some more.
+struct EnsShape {
+    uint32_t ncap, cap, lds;
+};
+
+EnsShape ens_shape(const oxdna_ctx* c, int nt) {
+    EnsShape s{};
+    s.ncap = std::max<uint32_t>(8u, (c->rep_nmax + 7u) & ~7u);
+    const uint32_t ncu = std::max<uint32_t>(1u, c->ncu);
+    const uint32_t waves = (uint32_t)nt / 64u;
+    const uint32_t need = s.ncap + 64u;
+    const uint32_t pairs = s.ncap * (s.ncap - 1u) / 2u;
+    const uint32_t base = ens_layout(s.ncap, 0u, (uint32_t)nt).bytes;
+    uint32_t k = std::min<uint32_t>((c->n_rep + ncu - 1u) / ncu, std::max<uint32_t>(1u, 8u / waves));
+    for (k = std::max<uint32_t>(k, 1u); k >= 1u; --k) {
+        const uint32_t budget = std::min<uint32_t>(c->lds_block, c->lds_cu / k);
+        if (base + 48u * need > budget) continue;
+        s.cap = std::min<uint32_t>((budget - base) / 48u, std::max(need, pairs));
+        s.lds = ens_layout(s.ncap, s.cap, (uint32_t)nt).bytes;
+        return s;
     }
-    throw Fail("internal: unsupported replica workgroup size");
+    throw Fail("replicas of " + std::to_string(c->rep_nmax) + " particles do not fit the LDS of one workgroup (" +
+               std::to_string(c->lds_block) + " bytes)");
 }
 
 // Launches the replica kernel with workgroup size nt (count: the counting variant). With
-// per-type-pair parameter sets the general instantiation (512 particles, 512 threads)
-// runs every replica.
-void launch_ens(oxdna_ctx* c, int nt, bool count, const EnsArgs& a) {
-    const int nmax = ens_nmax(c);
-    if (nt < nmax || a.n_rep == 0 || !a.offsets || !a.order || !a.P || c->rep_nmax > (uint32_t)nmax)
+// per-type-pair parameter sets, and for the count, the 512-thread instantiation runs.
+void launch_ens(oxdna_ctx* c, int nt, bool count, EnsArgs a) {
+    if (c->P.xwords || count) nt = 512;
+    if (nt < ens_nmax(c) || a.n_rep == 0 || !a.offsets || !a.order || !a.P || c->rep_nmax > (uint32_t)nt)
         throw Fail("internal: replica kernel launch does not fit the replicas");
-    dim3 g(a.n_rep);
-    if (c->P.xwords) {
-        if (count) hipLaunchKernelGGL((k_ensemble<512, 512, true, true>), g, dim3(512), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_ensemble<512, 512, false, true>), g, dim3(512), 0, c->stream, a);
-        return;
-    }
-    if (count) {
-        hipLaunchKernelGGL((k_ensemble<512, 512, true>), g, dim3(512), 0, c->stream, a);
-        return;
-    }
-    switch (nmax) {
-    case 64: launch_ens_nt<64>(nt, g, c->stream, a); break;
-    case 128: launch_ens_nt<128>(nt, g, c->stream, a); break;
-    case 256: launch_ens_nt<256>(nt, g, c->stream, a); break;
-    case 512: launch_ens_nt<512>(nt, g, c->stream, a); break;
-    default: throw Fail("internal: unsupported replica capacity");
-    }
+    const EnsShape sh = ens_shape(c, nt);
+    a.ncap = sh.ncap;
+    a.cap = sh.cap;
+    HIP_OK(launch_ensemble(nt, count, c->P.xwords != 0, a.n_rep, sh.lds, c->stream, a));
 }
 
 int ens_default_nt(const oxdna_ctx* c) { return std::max(256, ens_nmax(c)); }
@@ -2067,6 +2074,11 @@ int oxdna_ctx_create(int device, void* stream, oxdna_ctx** out) {
         int ncu = 0;
         HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
         c->ncu = (uint32_t)ncu;
+        int lds_block = 0, lds_cu = 0;
+        HIP_OK(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+        HIP_OK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device));
+        c->lds_block = (uint32_t)lds_block;
+        c->lds_cu = (uint32_t)std::max(lds_cu, lds_block);
         if (stream) c->stream = (hipStream_t)stream;
         else {
             HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));

@@ -690,8 +690,10 @@ __device__ __forceinline__ float stack_rna(const DBonded& P, V Rc, const Frame& 
 // Full bond (reference src/OxDNABondedKernel.h:357-431): FENE with FP64 arithmetic on
 // the FP64 minimum-image vector Rc = r_B - r_A, then bonded excluded volume and
 // stacking in FP32.
+// energy = false: forces and torques only (the FENE energy, an FP64 logarithm, is left
+// at 0 unless the bond is broken).
 __device__ __forceinline__ void bond_eval(const DBonded& P, double Rx, double Ry, double Rz, const Frame& fA,
-                                          const Frame& fB, int tA, int tB, BondRes& o) {
+                                          const Frame& fB, int tA, int tB, BondRes& o, bool energy = true) {
     o.FB = zero3(); o.TA = zero3(); o.TB = zero3();
     o.e_fene = 0.0; o.e_bexc = 0.f; o.e_stack = 0.f;
 #pragma unroll
@@ -705,7 +707,7 @@ __device__ __forceinline__ void bond_eval(const DBonded& P, double Rx, double Ry
     if (!(fabs(d) < P.delta_backbone)) {
         o.e_fene = 1.0e12;  // broken bond: energy sentinel, no force
     } else {
-        o.e_fene = -0.5 * P.eps_backbone * log(1.0 - d * d / P.delta2);
+        if (energy) o.e_fene = -0.5 * P.eps_backbone * log(1.0 - d * d / P.delta2);
         double pre = -(P.eps_backbone * d) / ((P.delta2 - d * d) * rm);
         V f = mk((float)(pre * rx), (float)(pre * ry), (float)(pre * rz));
         o.FB += f;

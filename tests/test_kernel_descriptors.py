@@ -20,7 +20,12 @@ from oxdna import _engine
 LLVM = "/opt/rocm/llvm/bin"
 
 
-def _code_object(tmp_path):
+MAGIC = b"__CLANG_OFFLOAD_BUNDLE__"
+
+
+def _code_objects(tmp_path):
+    """The gfx950 code object of every offload bundle in the library's fat-binary
+    section (one per translation unit with device code)."""
     lib = _engine.LIB_PATH
     if not os.path.exists(lib):
         pytest.fail(f"{lib} is missing: build it first")
@@ -29,20 +34,35 @@ def _code_object(tmp_path):
             pytest.skip(f"{tool} not available")
     if not os.path.exists(os.path.join(LLVM, "clang-offload-bundler")):
         pytest.skip("clang-offload-bundler not available")
-    fatbin, co = tmp_path / "fatbin.bin", tmp_path / "gfx950.co"
+    fatbin = tmp_path / "fatbin.bin"
     subprocess.check_call(["objcopy", "--dump-section", f".hip_fatbin={fatbin}", lib])
-    subprocess.check_call([os.path.join(LLVM, "clang-offload-bundler"), "--type=o", f"--input={fatbin}",
-                           f"--output={co}", "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", "--unbundle"])
-    return co
+    data = fatbin.read_bytes()
+    starts = []
+    k = data.find(MAGIC)
+    while k >= 0:
+        starts.append(k)
+        k = data.find(MAGIC, k + len(MAGIC))
+    assert starts and starts[0] == 0
+    out = []
+    for n, (a, b) in enumerate(zip(starts, starts[1:] + [len(data)])):
+        part, co = tmp_path / f"bundle{n}.bin", tmp_path / f"gfx950_{n}.co"
+        part.write_bytes(data[a:b])
+        subprocess.check_call([os.path.join(LLVM, "clang-offload-bundler"), "--type=o", f"--input={part}",
+                               f"--output={co}", "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", "--unbundle"])
+        out.append(co)
+    return out
 
 
-def test_every_kernel_has_hidden_arguments(tmp_path):
-    co = _code_object(tmp_path)
+def _kernels(co):
     notes = subprocess.run([os.path.join(LLVM, "llvm-readelf"), "--notes", str(co)], capture_output=True,
                            text=True, check=True).stdout
     i = notes.index("---")
     j = notes.index("...", i)
-    kernels = yaml.safe_load(notes[i:j])["amdhsa.kernels"]
+    return yaml.safe_load(notes[i:j])["amdhsa.kernels"]
+
+
+def test_every_kernel_has_hidden_arguments(tmp_path):
+    kernels = [k for co in _code_objects(tmp_path) for k in _kernels(co)]
     assert len(kernels) > 20
     missing = []
     for k in kernels:
