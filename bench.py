@@ -17,6 +17,7 @@ Prints one JSON line (see the fields below). Options:
     --gpus N           only 1 is supported (one GPU)
     --steps K          timed steps (default 10000)
     --warmup W         untimed steps before timing (default 1000)
+    --buffer B         pin the plain run's neighbour-list skin (default: autotuned)
     --no-cpu           skip the CPU baseline leg
     --no-batched       skip the batched ensembles
     --batched-steps S  timed steps of each ensemble (default 2000)
@@ -154,8 +155,7 @@ def bench_batched(name, reps, forces, kT, steps, warmup, census, bias=None, samp
     assert lib.oxdna_count_evaluations(ctx, ens._mask, counts) == 0
     flops = flops_per_launch(list(counts), census)
     achieved = flops / (avg_ms.value * 1e-3) / 1e12
-    tpp, nt = C.c_int(), C.c_int()
-    assert lib.oxdna_get_tuning(ctx, C.byref(tpp), C.byref(nt)) == 0
+    tuning = ens.tuning()
     # plain path, one replica at a time
     pick = np.linspace(0, n_rep - 1, min(sample, n_rep)).round().astype(int) if sample > 0 else []
     plain_steps, plain_time = 0, 0.0
@@ -183,8 +183,10 @@ def bench_batched(name, reps, forces, kT, steps, warmup, census, bias=None, samp
         "speedup_vs_plain": round(rate / plain_rate, 2) if plain_rate else None,
         "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 6), "traffic": None,
-                     "kernel": f"k_ensemble<{nt.value}, false, false> (forces only)", "kernel_ms": round(avg_ms.value, 5),
-                     "flops_per_launch": flops},
+                     "kernel": f"k_ensemble<{tuning['replica_workgroup']}, false, false> (forces only)",
+                     "kernel_ms": round(avg_ms.value, 5), "flops_per_launch": flops},
+        "tuning": {"replica_workgroup": tuning["replica_workgroup"],
+                   "replica_records_per_pass": tuning["replica_records_per_pass"]},
         "energy_per_nt": {k: round(float(v) / int(sizes.sum()), 6)
                           for k, v in zip(ensemble.REPLICA_ENERGY_NAMES, e.sum(axis=0))},
     }
@@ -259,6 +261,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--buffer", type=float, default=0.0,
+                    help="neighbour-list skin of the plain run (0: autotune over 0.1 ... 0.8)")
     ap.add_argument("--lanes", type=int, default=0,
                     help="force-kernel lanes per particle (4, 8, 16, 32) for the plain run; 0: autotune")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched replica workloads")
@@ -293,12 +297,17 @@ def main():
     from oxdna.model import dna1, dna2
 
     system = systems.duplex_gas(200, 20, box=200.0, seed=1)
-    forces, _ = dna2.forces(fused=True)
+    forces, nl = dna2.forces(fused=True)
+    if args.buffer > 0:
+        nl.buffer = args.buffer
     s = sim.Simulation(device=sim.GPU(0), seed=1, use_graph=not args.no_graph, lanes_per_particle=args.lanes)
     s.create_state_from_snapshot(system.snapshot())
     s.operations.integrator = sim.Integrator(
         dt=0.003, forces=forces, methods=[sim.Langevin(kT=dna1.kT_from_temperature(296.15))])
     s.run(max(args.warmup, 1))
+    # the neighbour-list skin from the autotuner (buffers 0.1 ... 0.8, 2000 timed steps
+    # each from the warmed-up state, which it restores), unless --buffer pins it
+    skin_rates = None if args.buffer > 0 else s.tune_nlist_buffer(steps=2000)
     s.synchronize()
     try:
         import torch
@@ -356,6 +365,8 @@ def main():
         "config": {"workload": "oxDNA2 200x20-bp duplexes, 8000 nt, box 200, Langevin dt=0.003, "
                                "fused HB+CS, Debye-Hueckel 0.5 M, 296.15 K",
                    "n_nucleotides": system.n, "graph": not args.no_graph},
+        "tuning": dict(s.tuning(), nlist_buffer_steps_per_s=(
+            {str(k): round(v, 1) for k, v in skin_rates.items()} if skin_rates else "pinned by --buffer")),
         "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 6),
                      "traffic": traffic,
@@ -388,7 +399,7 @@ def main():
                            census_rna, bias=ensemble.HarmonicBias(ends, kb, d0), sample=args.plain_sample)
         line["batched"] = [b1, b2]
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(system, forces, buffer=0.4)
+        line["cpu_baseline"] = cpu_baseline(system, forces, buffer=nl.buffer)
     print(json.dumps(line), flush=True)
 
 

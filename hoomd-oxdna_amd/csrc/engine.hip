@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1053,6 +1054,7 @@ struct oxdna_ctx {
     DBuf<EnsBias> rep_bias;
     bool have_bias = false;
     int ens_nt = 0;                      // workgroup size of the replica kernel (autotuned)
+    uint32_t ens_div = 1;                // its tile depth: the LDS share's record capacity / ens_div (autotuned)
     uint32_t ens_tuned_mask = 0xffffffffu;
 
     ~oxdna_ctx() {
@@ -1861,6 +1863,78 @@ void run(oxdna_ctx* c, uint32_t nsteps, uint32_t mask, int use_graph) {
     c->host_step += nsteps;
 }
 
+// Neighbour-list skin autotuner: from the current state, runs `warm` untimed and then
+// `steps` timed Langevin steps at each candidate buffer (rows rebuilt at that skin, cell
+// and row capacities sized afresh, lanes per particle tuned again), measures steps/s on
+// the host clock around whole run calls (rebuild launches, chunk ends and all), and
+// keeps the fastest. The state (positions, orientations, momenta, the stored
+// half-kick terms, the step count) is put back after every trial and at the end, so
+// the caller's trajectory continues from where it stood, at the chosen skin.
+double tune_buffer(oxdna_ctx* c, uint32_t mask, const std::vector<double>& cand, uint32_t steps, uint32_t warm,
+                   double* rates) {
+    if (c->n_rep) throw Fail("oxdna_tune_nlist_buffer: one-system contexts only (replicas have no neighbour list)");
+    if (!c->have_langevin) throw Fail("oxdna_tune_nlist_buffer: no integrator: call oxdna_set_langevin first");
+    if (cand.empty()) throw Fail("oxdna_tune_nlist_buffer: no candidate buffers");
+    for (double b : cand)
+        if (!(b > 0.0) || !(b <= 4.0)) throw Fail("oxdna_tune_nlist_buffer: candidate buffers must lie in (0, 4]");
+    if (steps == 0) throw Fail("oxdna_tune_nlist_buffer: steps must be > 0");
+    ensure_ready(c);
+    const uint32_t m = effective_mask(c, mask);
+    if (!(m & 0x3fu)) throw Fail("oxdna_tune_nlist_buffer: no nonbonded term selected (no neighbour list to tune)");
+    if (c->forces_mask != m || !c->nlist_valid) compute(c, m, 1);
+    // the starting state, kept apart from the run's own per-chunk save
+    DBuf<double4> t_pos, t_vel, t_quat, t_ang, t_last, t_bd, t_bdtq;
+    DBuf<float4> t_f, t_t;
+    t_pos.alloc(c->N); t_vel.alloc(c->N); t_quat.alloc(c->N); t_ang.alloc(c->N); t_last.alloc(c->N);
+    t_bd.alloc(c->N); t_bdtq.alloc(c->N); t_f.alloc(c->N); t_t.alloc(c->N);
+    const RunState keep{t_pos.p, t_vel.p, t_quat.p, t_ang.p, t_last.p, t_bd.p, t_bdtq.p, t_f.p, t_t.p};
+    hipLaunchKernelGGL(k_copy_state, dim3(grid_stride_blocks(c->N)), dim3(kBlock), 0, c->stream, c->N,
+                       live_state(c), keep);
+    HIP_OK(hipGetLastError());
+    const uint64_t step_kept = c->host_step;
+    DevFlags f0;
+    read_flags(c, f0);
+    auto set_skin = [&](double b) {
+        hipLaunchKernelGGL(k_copy_state, dim3(grid_stride_blocks(c->N)), dim3(kBlock), 0, c->stream, c->N, keep,
+                           live_state(c));
+        HIP_OK(hipGetLastError());
+        c->host_step = step_kept;
+        c->buffer = b;
+        c->params_dirty = true;
+        c->nlist_valid = false;
+        c->max_neigh = 0;   // capacities sized afresh for this skin
+        c->cell_cap = 0;
+        c->tuned_mask = 0xffffffffu;
+        compute(c, m, 1);
+    };
+    double best = -1.0, best_b = cand[0];
+    for (size_t k = 0; k < cand.size(); ++k) {
+        set_skin(cand[k]);
+        if (warm) run(c, warm, m, 1);
+        HIP_OK(hipStreamSynchronize(c->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        run(c, steps, m, 1);
+        HIP_OK(hipStreamSynchronize(c->stream));
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        const double rate = steps / std::max(sec, 1e-9);
+        if (rates) rates[k] = rate;
+        if (rate > best) { best = rate; best_b = cand[k]; }
+    }
+    set_skin(best_b);
+    autotune(c, m);
+    build_graph(c, m);  // the caller's next run replays it instead of capturing it
+    // the step and build counts as they stood (the trials' steps are not the caller's)
+    DevFlags f;
+    read_flags(c, f);
+    f.step = f0.step;
+    f.builds = f0.builds;
+    HIP_OK(hipMemcpyAsync(c->flags.p, &f, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    t_pos.release(); t_vel.release(); t_quat.release(); t_ang.release(); t_last.release();
+    t_bd.release(); t_bdtq.release(); t_f.release(); t_t.release();
+    return best_b;
+}
+
 // ------------------------------------------------------------------ batched replicas
 constexpr uint32_t kEnsMaxN = 512;     // largest replica (particles), one workgroup each
 constexpr uint32_t kEnsChunk = 2048;   // Langevin steps per persistent launch
@@ -1940,15 +2014,16 @@ int ens_nmax(const oxdna_ctx* c) {
 }
 
 // LDS shape of a replica launch on nt-thread workgroups: particle capacity (the largest
-// replica, rounded up to 8) and pair records per pass, as many as fit the LDS share of
-// one workgroup when ceil(replicas / compute units) workgroups share a compute unit (at
-// most 8 wavefronts per compute unit: the kernel's registers allow 2 per SIMD). At least
-// ncap + 64 records, so a pass holds any particle's entries and some more.
+// replica, rounded up to 8) and pair records per pass (the tile depth), as many as fit
+// the LDS share of one workgroup when ceil(replicas / compute units) workgroups share a
+// compute unit (at most 8 wavefronts per compute unit: the kernel's registers allow 2
+// per SIMD), divided by `div` (1, 2, 4: shallower tiles, more passes, less LDS). At
+// least ncap + 64 records, so a pass holds any particle's entries and some more.
 struct EnsShape {
     uint32_t ncap, cap, lds;
 };
 
-EnsShape ens_shape(const oxdna_ctx* c, int nt) {
+EnsShape ens_shape(const oxdna_ctx* c, int nt, uint32_t div = 1) {
     EnsShape s{};
     s.ncap = std::max<uint32_t>(8u, (c->rep_nmax + 7u) & ~7u);
     const uint32_t ncu = std::max<uint32_t>(1u, c->ncu);
@@ -1961,6 +2036,7 @@ EnsShape ens_shape(const oxdna_ctx* c, int nt) {
         const uint32_t budget = std::min<uint32_t>(c->lds_block, c->lds_cu / k);
         if (base + 48u * need > budget) continue;
         s.cap = std::min<uint32_t>((budget - base) / 48u, std::max(need, pairs));
+        s.cap = std::max(need, s.cap / std::max<uint32_t>(div, 1u));
         s.lds = ens_layout(s.ncap, s.cap, (uint32_t)nt).bytes;
         return s;
     }
@@ -1974,7 +2050,7 @@ void launch_ens(oxdna_ctx* c, int nt, bool count, EnsArgs a) {
     if (c->P.xwords || count) nt = 512;
     if (nt < ens_nmax(c) || a.n_rep == 0 || !a.offsets || !a.order || !a.P || c->rep_nmax > (uint32_t)nt)
         throw Fail("internal: replica kernel launch does not fit the replicas");
-    const EnsShape sh = ens_shape(c, nt);
+    const EnsShape sh = ens_shape(c, nt, (c->P.xwords || count) ? 1u : c->ens_div);
     a.ncap = sh.ncap;
     a.cap = sh.cap;
     HIP_OK(launch_ensemble(nt, count, c->P.xwords != 0, a.n_rep, sh.lds, c->stream, a));
@@ -1983,36 +2059,47 @@ void launch_ens(oxdna_ctx* c, int nt, bool count, EnsArgs a) {
 int ens_default_nt(const oxdna_ctx* c) { return std::max(256, ens_nmax(c)); }
 
 // Autotuner: workgroup size of the replica kernel (from the replica capacity up to
-// 512 threads, every power of two), timed as forces-only launches (the state is not
-// changed) with HIP events; keeps the fastest.
+// 512 threads, every power of two) and its tile depth (pair records per pass: the LDS
+// share's full depth, a half or a quarter of it, each distinct depth once), timed as
+// forces-only launches (the state is not changed) with HIP events; keeps the fastest.
 void autotune_ens(oxdna_ctx* c, uint32_t m) {
     if (c->P.xwords) {  // the general instantiation has one workgroup size
         c->ens_nt = 512;
+        c->ens_div = 1;
         c->ens_tuned_mask = m;
         return;
     }
     if (c->ens_tuned_mask == m && c->ens_nt != 0) return;
     float best = 1e30f;
     int best_nt = ens_default_nt(c);
+    uint32_t best_div = 1;
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
     for (int nt = ens_nmax(c); nt <= 512; nt *= 2) {
-        EnsArgs a = ens_args(c, m, 0, 8, 0);
-        launch_ens(c, nt, false, a);
-        for (int trial = 0; trial < 3; ++trial) {
-            HIP_OK(hipEventRecord(e0, c->stream));
+        uint32_t last_cap = 0;
+        for (uint32_t div = 1; div <= 4; div *= 2) {
+            const uint32_t cap = ens_shape(c, nt, div).cap;
+            if (cap == last_cap) continue;  // the depth is at its floor already
+            last_cap = cap;
+            c->ens_div = div;
+            EnsArgs a = ens_args(c, m, 0, 8, 0);
             launch_ens(c, nt, false, a);
-            HIP_OK(hipEventRecord(e1, c->stream));
-            HIP_OK(hipEventSynchronize(e1));
-            float ms = 0.f;
-            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best) { best = ms; best_nt = nt; }
+            for (int trial = 0; trial < 3; ++trial) {
+                HIP_OK(hipEventRecord(e0, c->stream));
+                launch_ens(c, nt, false, a);
+                HIP_OK(hipEventRecord(e1, c->stream));
+                HIP_OK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) { best = ms; best_nt = nt; best_div = div; }
+            }
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->ens_nt = best_nt;
+    c->ens_div = best_div;
     c->ens_tuned_mask = m;
 }
 
@@ -2469,6 +2556,7 @@ int oxdna_set_replicas(oxdna_ctx* c, unsigned int n_replicas, const unsigned int
         c->rep_nmax = nmax;
         c->have_bias = false;
         c->ens_nt = 0;
+        c->ens_div = 1;
         c->ens_tuned_mask = 0xffffffffu;
         c->forces_mask = 0xffffffffu;
         c->drop_graph();
@@ -2656,6 +2744,32 @@ int oxdna_get_tuning(oxdna_ctx* c, int* lanes_per_particle, int* replica_workgro
     return guard(c, [&] {
         if (lanes_per_particle) *lanes_per_particle = c->tpp;
         if (replica_workgroup) *replica_workgroup = c->ens_nt;
+    });
+}
+
+int oxdna_get_tuning_ex(oxdna_ctx* c, oxdna_tuning* out) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (!out) throw Fail("oxdna_get_tuning_ex: out is NULL");
+        out->lanes_per_particle = c->tpp;
+        out->particles_per_workgroup = kBlock / c->tpp;
+        out->replica_workgroup = c->ens_nt;
+        out->replica_records_per_pass = (c->n_rep && c->ens_nt) ? (int)ens_shape(c, c->ens_nt, c->ens_div).cap : 0;
+        out->nlist_buffer = c->buffer;
+    });
+}
+
+int oxdna_tune_nlist_buffer(oxdna_ctx* c, unsigned int mask, const double* buffers, unsigned int n_buffers,
+                            unsigned int steps, double* steps_per_s, double* chosen) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        std::vector<double> cand;
+        if (buffers) cand.assign(buffers, buffers + n_buffers);
+        else
+            for (int k = 1; k <= 8; ++k) cand.push_back(0.1 * k);
+        const double b = tune_buffer(c, mask, cand, steps, std::min<uint32_t>(std::max<uint32_t>(steps / 4, 1), 400),
+                                     steps_per_s);
+        if (chosen) *chosen = b;
     });
 }
 

@@ -104,6 +104,13 @@ class Box(C.Structure):
     _fields_ = [("L", D3)]
 
 
+class Tuning(C.Structure):
+    """oxdna_tuning: the autotuner's launch parameters (no reference counterpart)."""
+    _fields_ = [("lanes_per_particle", C.c_int), ("particles_per_workgroup", C.c_int),
+                ("replica_workgroup", C.c_int), ("replica_records_per_pass", C.c_int),
+                ("nlist_buffer", C.c_double)]
+
+
 PARAM_STRUCT = {
     TERM_EXCVOL: ExcVolParams,
     TERM_HBOND: HBondParams,

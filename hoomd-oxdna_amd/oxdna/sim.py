@@ -310,6 +310,38 @@ class Simulation:
     def synchronize(self):
         self._check(self._lib.oxdna_synchronize(self._ctx), "synchronize")
 
+    def tune_nlist_buffer(self, buffers=None, steps=2000):
+        """Neighbour-list skin autotuner (the reference fixes buffer 0.4,
+        oxdna/model/dna2.py:207-208). From the current state, runs `steps` Langevin
+        steps at each candidate buffer (default 0.1, 0.2, ..., 0.8), keeps the one with
+        the most steps/s, sets it on the attached neighbour lists and restores the state
+        and timestep. Returns {buffer: steps/s}. Tune on an equilibrated state: the
+        rebuild rate it measures is that state's."""
+        if not self._attached:
+            self._attach()
+        if not self.operations.integrator.methods:
+            raise RuntimeError("tuning the skin runs steps: attach an integration method (methods.Langevin)")
+        cand = None if buffers is None else np.ascontiguousarray([float(b) for b in buffers], dtype=np.float64)
+        n = 8 if cand is None else len(cand)
+        rates = np.zeros(n)
+        chosen = C.c_double()
+        dp = C.POINTER(C.c_double)
+        self._check(self._lib.oxdna_tune_nlist_buffer(
+            self._ctx, self._mask, None if cand is None else cand.ctypes.data_as(dp), n, int(steps),
+            rates.ctypes.data_as(dp), C.byref(chosen)), "tune_nlist_buffer")
+        for f in self.operations.integrator.forces:
+            if isinstance(f, AnisotropicPair):
+                f.nlist.buffer = chosen.value
+        self._cache.clear()
+        grid = [0.1 * k for k in range(1, 9)] if cand is None else list(cand)
+        return {round(float(b), 6): float(r) for b, r in zip(grid, rates)}
+
+    def tuning(self):
+        """The autotuner's current launch parameters and skin."""
+        t = _abi.Tuning()
+        self._check(self._lib.oxdna_get_tuning_ex(self._ctx, C.byref(t)), "get_tuning_ex")
+        return {name: getattr(t, name) for name, _ in t._fields_}
+
     def stats(self):
         nb, mx, st = C.c_uint64(), C.c_uint32(), C.c_uint64()
         self._check(self._lib.oxdna_get_stats(self._ctx, C.byref(nb), C.byref(mx), C.byref(st)), "get_stats")

@@ -351,6 +351,35 @@ int oxdna_set_lanes_per_particle(oxdna_ctx* ctx, int lanes);
  * workgroup size of the replica kernel (64..512; 0 before the first replica run). */
 int oxdna_get_tuning(oxdna_ctx* ctx, int* lanes_per_particle, int* replica_workgroup);
 
+/* Every autotuned launch parameter of a context (oxdna_get_tuning_ex). */
+typedef struct oxdna_tuning {
+    int lanes_per_particle;       /* force kernel: lanes per particle (4, 8, 16, 32) */
+    int particles_per_workgroup;  /* force kernel: particles per 256-thread workgroup (its tile) */
+    int replica_workgroup;        /* replica kernel: threads per workgroup (0 before the first replica run) */
+    int replica_records_per_pass; /* replica kernel: pair records per pass, its tile depth (0: none yet) */
+    double nlist_buffer;          /* neighbour-list skin in use (oxdna_set_nlist or the skin autotuner) */
+} oxdna_tuning;
+/* The autotuner's current choices. Block size and tile depth are timed at the first run
+ * of a setup (the force kernel's lanes per particle; the replica kernel's workgroup size
+ * and records per pass); the skin by oxdna_tune_nlist_buffer. The reference tunes its
+ * kernels' block sizes with HOOMD's Autotuner (src/OxDNABondedForceComputeGPU.cc:35-37,
+ * 91-100) and fixes the skin (buffer 0.4, oxdna/model/dna2.py:207-208). */
+int oxdna_get_tuning_ex(oxdna_ctx* ctx, oxdna_tuning* out);
+/* Neighbour-list skin autotuner (one-system contexts with an integrator and at least
+ * one nonbonded term in term_mask). From the current state, runs min(steps / 4, 400)
+ * untimed and then `steps` timed Langevin steps at each of the n_buffers candidate
+ * buffers (each in (0, 4]; buffers = NULL: 0.1, 0.2, ..., 0.8, n_buffers ignored),
+ * rows and capacities built afresh for each, and keeps the one with the most steps/s
+ * (host clock around whole runs: rebuilds and chunk ends included). steps_per_s
+ * (NULL or n_buffers doubles, 8 for the default set) receives each candidate's rate,
+ * *chosen (may be NULL) the kept buffer, which stays set as by oxdna_set_nlist. The
+ * state and step count are restored afterwards: the trajectory continues from the
+ * state it had, at the chosen skin. Call it on an equilibrated state: the rebuild rate
+ * it measures is that state's. */
+int oxdna_tune_nlist_buffer(oxdna_ctx* ctx, unsigned int term_mask, const double* buffers,
+                            unsigned int n_buffers, unsigned int steps, double* steps_per_s,
+                            double* chosen);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,6 +23,7 @@ STRUCTS = {
     "oxdna_debye_params": _abi.DebyeParams,
     "oxdna_bonded_params": _abi.BondedParams,
     "oxdna_box": _abi.Box,
+    "oxdna_tuning": _abi.Tuning,
 }
 
 
