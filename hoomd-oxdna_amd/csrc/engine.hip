@@ -2777,43 +2777,58 @@ int oxdna_tune_nlist_buffer(oxdna_ctx* c, unsigned int mask, const double* buffe
 
 // ------------------------------------------------------------------ stateless kernel drivers
 namespace {
-// Library-owned device scratch of the stateless drivers, one set per device (grown on
-// demand, kept until the process ends). Calls that overlap on one device race on it.
-struct DriverScratch {
-    DBuf<float4> fr0, fr1, fr2;
-    DBuf<uint4> P;  // parameter block: DParams, then its extras
-    DBuf<double> epart;
-    DBuf<int> bpart, binfo;
-    DBuf<uint32_t> err, zeros;
-    DBuf<size_t> zhead;
-    // host copies of the last parameter blocks: an asynchronous copy reads its source
-    // after the call returns
-    std::vector<std::vector<uint4>> staged = std::vector<std::vector<uint4>>(8);
-    int ring = 0;
+// Device workspace of one stateless driver call (parameter block, frames, energy
+// partials, bond tables): one stream-ordered allocation on the caller's stream from a
+// memory pool the library keeps per device (its memory stays reserved between calls,
+// so an allocation after the first costs no device allocation), returned to the pool
+// on the same stream when the call's launches are enqueued. Calls share no device
+// memory: concurrent calls on one device, from other streams or threads, are safe.
+hipMemPool_t g_pool[64] = {};
+std::mutex g_pool_mu;
+
+hipMemPool_t driver_pool(int dev) {
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    if (!g_pool[dev]) {
+        hipMemPoolProps props{};
+        props.allocType = hipMemAllocationTypePinned;
+        props.handleTypes = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t pool = nullptr;
+        HIP_OK(hipMemPoolCreate(&pool, &props));
+        uint64_t keep = ~0ull;  // never trim the pool at synchronisation
+        HIP_OK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+        g_pool[dev] = pool;
+    }
+    return g_pool[dev];
+}
+
+struct Workspace {
+    unsigned char* p = nullptr;
+    hipStream_t st = nullptr;
+    size_t used = 0;
+    Workspace(size_t bytes, hipStream_t stream) : st(stream) {
+        int dev = 0;
+        HIP_OK(hipGetDevice(&dev));
+        if (dev < 0 || dev >= 64) throw Fail("device index out of range");
+        HIP_OK(hipMallocFromPoolAsync((void**)&p, std::max<size_t>(bytes, 256), driver_pool(dev), st));
+    }
+    ~Workspace() {
+        if (p) (void)hipFreeAsync(p, st);
+    }
+    Workspace(const Workspace&) = delete;
+    Workspace& operator=(const Workspace&) = delete;
+    // the next 256-byte-aligned piece of n elements of T
+    template<class T> T* take(size_t n) {
+        T* q = reinterpret_cast<T*>(p + used);
+        used += (n * sizeof(T) + 255) & ~size_t(255);
+        return q;
+    }
 };
-DriverScratch g_scratch[64];
 
-DriverScratch& scratch_for(int& dev) {
-    HIP_OK(hipGetDevice(&dev));
-    if (dev < 0 || dev >= 64) throw Fail("device index out of range");
-    return g_scratch[dev];
-}
+constexpr size_t ws_bytes(size_t n, size_t elem) { return (n * elem + 255) & ~size_t(255); }
 
-const DParams* stage_params(DriverScratch& s, std::vector<uint4> blob, hipStream_t st) {
-    s.P.alloc(blob.size());
-    std::vector<uint4>& h = s.staged[s.ring];
-    s.ring = (s.ring + 1) % (int)s.staged.size();
-    h = std::move(blob);
-    HIP_OK(hipMemcpyAsync(s.P.p, h.data(), sizeof(uint4) * h.size(), hipMemcpyHostToDevice, st));
-    return reinterpret_cast<const DParams*>(s.P.p);
-}
-
-void driver_frames(DriverScratch& s, uint32_t N, const double* d_pos, const double* d_orient, const double* d_charge,
-                   hipStream_t st) {
-    s.fr0.alloc(N); s.fr1.alloc(N); s.fr2.alloc(N);
-    hipLaunchKernelGGL(k_frames, dim3(blocks_for(N)), dim3(kBlock), 0, st, N, (const double4*)d_orient, d_charge,
-                       (const double4*)d_pos, s.fr0.p, s.fr1.p, s.fr2.p);
-}
+void put_params(const std::vector<uint4>& blob, uint4* dst, hipStream_t st);
 
 // Lanes per particle of the stateless drivers' force launches.
 constexpr int kDriverTpp = 8;
@@ -2974,13 +2989,17 @@ int oxdna_pair_forces(int term, float* d_force, float* d_torque, float* d_virial
         if (N == 0) return;
         const uint32_t xwords = reinterpret_cast<const DParams*>(blob.data())->xwords;
         hipStream_t st = (hipStream_t)stream;
-        int dev;
-        DriverScratch& s = scratch_for(dev);
-        const DParams* dP = stage_params(s, std::move(blob), st);
-        s.fr0.alloc(N); s.fr1.alloc(N); s.fr2.alloc(N);
-        s.epart.alloc((size_t)blocks_for((size_t)N * kDriverTpp) * 8);
+        const size_t nep = (size_t)blocks_for((size_t)N * kDriverTpp) * 8;
+        Workspace ws(ws_bytes(blob.size(), sizeof(uint4)) + 3 * ws_bytes(N, sizeof(float4)) +
+                         ws_bytes(nep, sizeof(double)), st);
+        uint4* dP = ws.take<uint4>(blob.size());
+        float4* fr0 = ws.take<float4>(N);
+        float4* fr1 = ws.take<float4>(N);
+        float4* fr2 = ws.take<float4>(N);
+        double* epart = ws.take<double>(nep);
+        put_params(blob, dP, st);
         pair_driver_launch(term, d_force, d_torque, d_virial, virial_pitch, N, d_pos, d_orientation, d_charge, box,
-                           d_n_neigh, d_nlist, d_head_list, dP, xwords, s.fr0.p, s.fr1.p, s.fr2.p, s.epart.p,
+                           d_n_neigh, d_nlist, d_head_list, (const DParams*)dP, xwords, fr0, fr1, fr2, epart,
                            d_term_energy, st);
     });
 }
@@ -3025,8 +3044,6 @@ int oxdna_bonded_forces(float* d_force, float* d_torque, float* d_virial, size_t
             throw Fail("oxdna_bonded_forces: NULL device array");
         if (N > kIndexMask) throw Fail("oxdna_bonded_forces: N must be below 2^30");
         hipStream_t st = (hipStream_t)stream;
-        int dev;
-        DriverScratch& s = scratch_for(dev);
         DParams P{};
         for (uint32_t k = 0; k < n_bond_types; ++k) load_bond_params(P.bond[k], params[k]);
         P.n_bond_types = (int)n_bond_types;
@@ -3034,40 +3051,53 @@ int oxdna_bonded_forces(float* d_force, float* d_torque, float* d_virial, size_t
         P.mask = 1u << OXDNA_TERM_BONDED;
         std::vector<uint4> blob(sizeof(DParams) / sizeof(uint4));
         std::memcpy(blob.data(), &P, sizeof(DParams));
-        const DParams* dP = stage_params(s, std::move(blob), st);
-        driver_frames(s, N, d_pos, d_orientation, nullptr, st);
-        s.bpart.alloc(2 * (size_t)N); s.binfo.alloc(2 * (size_t)N); s.err.alloc(1);
-        HIP_OK(hipMemsetAsync(s.err.p, 0, sizeof(uint32_t), st));
+        const int tpp = 8;
+        const size_t nep = (size_t)blocks_for((size_t)N * tpp) * 8;
+        // parameters, frames, bond tables, the error word with the empty rows' zeros after
+        // it (N + 1 words), zero row heads, energy partials
+        Workspace ws(ws_bytes(blob.size(), sizeof(uint4)) + 3 * ws_bytes(N, sizeof(float4)) +
+                         2 * ws_bytes(2 * (size_t)N, sizeof(int)) + ws_bytes((size_t)N + 1, sizeof(uint32_t)) +
+                         ws_bytes(N, sizeof(size_t)) + ws_bytes(nep, sizeof(double)), st);
+        uint4* dP = ws.take<uint4>(blob.size());
+        float4* fr0 = ws.take<float4>(N);
+        float4* fr1 = ws.take<float4>(N);
+        float4* fr2 = ws.take<float4>(N);
+        int* bpart = ws.take<int>(2 * (size_t)N);
+        int* binfo = ws.take<int>(2 * (size_t)N);
+        uint32_t* zeros = ws.take<uint32_t>((size_t)N + 1);  // [0]: error word, then N empty row lengths
+        size_t* zhead = ws.take<size_t>(N);
+        double* epart = ws.take<double>(nep);
+        put_params(blob, dP, st);
+        hipLaunchKernelGGL(k_frames, dim3(blocks_for(N)), dim3(kBlock), 0, st, N, (const double4*)d_orientation,
+                           (const double*)nullptr, (const double4*)d_pos, fr0, fr1, fr2);
+        // fill kernels, not memset nodes: a caller may capture the call into a graph
+        hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for((size_t)N + 1), 4096)), dim3(kBlock), 0, st,
+                           zeros, (size_t)N + 1);
+        hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for(2 * (size_t)N), 4096)), dim3(kBlock), 0, st,
+                           (uint32_t*)zhead, 2 * (size_t)N);
         hipLaunchKernelGGL(k_bond_table, dim3(blocks_for(N)), dim3(kBlock), 0, st, N, d_blist, d_bpos, pitch,
-                           d_n_bonds, s.bpart.p, s.binfo.p, s.err.p);
+                           d_n_bonds, bpart, binfo, zeros);
         ForceArgs a{};
         a.N = N;
         a.pos = (const double4*)d_pos;
-        a.fr0 = s.fr0.p; a.fr1 = s.fr1.p; a.fr2 = s.fr2.p;
-        a.bpart = s.bpart.p; a.binfo = s.binfo.p;
-        a.P = dP;
+        a.fr0 = fr0; a.fr1 = fr1; a.fr2 = fr2;
+        a.bpart = bpart; a.binfo = binfo;
+        a.P = (const DParams*)dP;
         for (int k = 0; k < 3; ++k) { a.L[k] = box->L[k]; a.Linv[k] = 1.0 / box->L[k]; }
         a.mask = 1u << OXDNA_TERM_BONDED;
         a.force = (float4*)d_force; a.torque = (float4*)d_torque;
         a.virial = d_virial; a.vpitch = virial_pitch;
         a.split = (double4*)d_split_energy;
         // no nonbonded neighbours: every row is empty
-        s.zeros.alloc(N);
-        s.zhead.alloc(N);
-        hipLaunchKernelGGL(k_zero_u32, dim3(std::min<size_t>(blocks_for(N), 4096)), dim3(kBlock), 0, st, s.zeros.p,
-                           (size_t)N);
-        HIP_OK(hipMemsetAsync(s.zhead.p, 0, sizeof(size_t) * N, st));
-        a.n_neigh = s.zeros.p;
-        a.nlist = s.zeros.p;
-        a.head = s.zhead.p;
-        const int tpp = 8;
-        s.epart.alloc((size_t)blocks_for((size_t)N * tpp) * 8);
-        a.epart = s.epart.p;
+        a.n_neigh = zeros + 1;
+        a.nlist = zeros + 1;
+        a.head = zhead;
+        a.epart = epart;
         // split energies need the energy instantiation
         launch_k_any(tpp, d_split_energy != nullptr, d_virial != nullptr, dim3(blocks_for((size_t)N * tpp)),
                      dim3(kBlock), 0, st, a);
         uint32_t herr = 0;
-        HIP_OK(hipMemcpyAsync(&herr, s.err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(&herr, zeros, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
         if (herr) throw Fail("oxdna_bonded_forces: a particle has more than two backbone bonds");
     });

@@ -171,6 +171,12 @@ int oxdna_device_name(int device, char* buf, size_t buflen);
 
 /* ------------------------------------------------------------------------- */
 /* (1) Stateless kernel drivers, HOOMD layouts, device pointers, one stream.  */
+/* Each call enqueues its launches on `stream` and takes its device workspace  */
+/* (parameter block, frames, energy partials) as a stream-ordered allocation   */
+/* from a memory pool the library keeps per device, returned to the pool on    */
+/* the same stream: calls share no device memory, so calls on one device from */
+/* several streams or threads may overlap. Errors of these context-free calls  */
+/* are reported per thread (oxdna_last_error(NULL)).                           */
 /* ------------------------------------------------------------------------- */
 
 /* Nonbonded pair forces for one term over a full neighbour list. Replaces the
