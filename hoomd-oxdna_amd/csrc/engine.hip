@@ -1936,7 +1936,9 @@ double tune_buffer(oxdna_ctx* c, uint32_t mask, const std::vector<double>& cand,
 }
 
 // ------------------------------------------------------------------ batched replicas
-constexpr uint32_t kEnsMaxN = 512;     // largest replica (particles), one workgroup each
+// Largest replica (particles), one workgroup each: its LDS layout with the minimum record
+// capacity (ncap + 64 pair records) fills at most the 160 KiB of one gfx950 workgroup.
+constexpr uint32_t kEnsMaxN = 496;
 constexpr uint32_t kEnsChunk = 2048;   // Langevin steps per persistent launch
 
 // Sums the per-replica energies into the context's 8 term totals (one block, fixed order).

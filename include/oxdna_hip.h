@@ -307,7 +307,7 @@ int oxdna_get_state(oxdna_ctx* ctx, double* pos, double* quat, double* vel, doub
  * independent systems. Replica r holds particles [offsets[r], offsets[r+1]) and lives
  * in its own periodic box boxes[3r..3r+2] (each length above twice the largest
  * cut-off); no pair or bond crosses replicas (bonds that would are rejected). Each
- * replica has at most 512 particles. oxdna_compute / oxdna_run / oxdna_get_forces /
+ * replica has at most 496 particles (its LDS layout fills one workgroup's 160 KiB). oxdna_compute / oxdna_run / oxdna_get_forces /
  * oxdna_get_state then act on all replicas at once, in one persistent launch per
  * chunk of steps (one workgroup per replica, its state resident in LDS); replica r
  * draws its Langevin noise with seed + r, i.e. exactly the stream of a one-system

@@ -107,8 +107,11 @@ def test_rna_cloverleaf_replicas_with_bias_match_oracle():
     assert np.all(e > 0) and e[-1] > e[0]
 
 
-def test_uneven_sizes_one_launch_up_to_512():
-    """Replicas from 1 to 512 nt in one launch (workgroup of 512 threads)."""
+def test_uneven_sizes_one_launch_up_to_496():
+    """Replicas from 8 to 496 nt in one launch (workgroup of 512 threads): 496 nt is the
+    largest replica whose LDS layout (state, frames, bond and pair tables, the minimum
+    of ncap + 64 pair records) fits the 160 KiB of one workgroup; a 497-nt replica is
+    rejected with the limit in the message."""
     rng = np.random.default_rng(2)
     big = systems.duplex_gas(12, 20, box=40.0, seed=4, min_dist=1.5)  # 480 nt
     single = systems.duplex_system(systems.random_sequence(rng, 4), box=20.0)
@@ -116,6 +119,14 @@ def test_uneven_sizes_one_launch_up_to_512():
     reps = [single, big, mid]
     forces, _ = dna2.forces(fused=True)
     _check_replicas(reps, forces, "uneven sizes")
+    # the largest accepted replica (31 duplexes of 8 bp: 496 nt) against the oracle
+    top = systems.duplex_gas(31, 8, box=40.0, seed=5, min_dist=1.5)
+    _check_replicas([top, single], forces, "496-nt replica")
+    # a larger one is refused (83 duplexes of 3 bp: 498 nt)
+    over = systems.duplex_gas(83, 3, box=60.0, seed=6, min_dist=1.5)
+    ens2 = ensemble.ReplicaEnsemble(device=sim.GPU(), seed=3)
+    with pytest.raises(Exception, match="at most 496"):
+        ens2.create_state_from_snapshots([over.snapshot()])
 
 
 def test_replicas_follow_one_system_runs():
