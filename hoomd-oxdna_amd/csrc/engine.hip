@@ -440,17 +440,20 @@ __device__ __forceinline__ uint32_t prefetch_args() {
 // half of step k and the first half of step k + 1 (a.integrate == 2); the force-free part
 // of that runs during the first round, and the force and torque, consumed in the
 // epilogue, are not stored. FAR2: a thread evaluates two far row entries of a particle
-// (grids too small to fill the GPU: fewer rounds). Register budget: the plain kernel at 3 waves per SIMD (168 VGPRs, no
-// spills: large systems are bound by issue rate across many waves); the fused one, used
-// only while the grid holds <= 2 waves per SIMD, and the single-point energy + virial
-// one, unconstrained.
+// (grids too small to fill the GPU: fewer rounds). Register budget: the plain kernel of
+// large grids at 4 waves per SIMD (128 VGPRs, no spills, built without machine-level
+// loop-invariant code motion: large systems are bound by issue rate across many waves,
+// which more waves per SIMD fill better); the other forces-only, energy and counting
+// kernels at 3 (168); the fused one, used only while the grid holds <= 2 waves per
+// SIMD, and the single-point energy + virial one, unconstrained.
 // MULTI: some term's table differs between type pairs (a.xwords != 0): each pair finds its
 // parameter sets through their offsets in the block, staged with the extras in dynamic
 // LDS. Otherwise the sets sit at fixed offsets of a static copy of DParams, where the
 // compiler merges their reads into 64- and 128-bit LDS loads (a set address chosen at run
 // time loses that: measured 3 % slower).
 template<int P, bool WANT_E, bool WANT_V, bool COUNT, bool FUSED = false, bool FAR2 = false, bool MULTI = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((FUSED || (WANT_E && WANT_V)) ? 1 : 3)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    (FUSED || (WANT_E && WANT_V)) ? 1 : ((FAR2 || WANT_E || WANT_V || COUNT || MULTI) ? 3 : 4))))
 void k_forces(ForceArgs a) {
     constexpr uint32_t TPP = 256 / P;
     // A grid of at most two workgroups per compute unit runs in two batches: blockIdx

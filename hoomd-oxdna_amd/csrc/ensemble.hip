@@ -1,8 +1,9 @@
 // ensemble.hip — the batched-replica kernel's instantiations and their launcher, in a
-// translation unit of their own, built without machine-level loop-invariant code motion:
-// with it, the constants and invariant products of the integrator and of the pair terms
-// are hoisted out of the step loop, both sets stay live across the whole loop, and the
-// kernel needs more than 256 VGPRs (one wavefront per SIMD); without it, under 240.
+// translation unit of their own (built, like engine.hip, without machine-level
+// loop-invariant code motion: with it, the constants and invariant products of the
+// integrator and of the pair terms are hoisted out of the step loop, both sets stay live
+// across the whole loop, and the kernel needs more than 256 VGPRs, one wavefront per
+// SIMD; without it, 254).
 #include "ensemble.h"
 
 namespace oxd {
