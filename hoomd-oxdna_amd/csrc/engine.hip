@@ -1885,16 +1885,25 @@ double tune_buffer(oxdna_ctx* c, uint32_t mask, const std::vector<double>& cand,
     const uint32_t m = effective_mask(c, mask);
     if (!(m & 0x3fu)) throw Fail("oxdna_tune_nlist_buffer: no nonbonded term selected (no neighbour list to tune)");
     if (c->forces_mask != m || !c->nlist_valid) compute(c, m, 1);
-    // the starting state, kept apart from the run's own per-chunk save
-    DBuf<double4> t_pos, t_vel, t_quat, t_ang, t_last, t_bd, t_bdtq;
-    DBuf<float4> t_f, t_t;
-    t_pos.alloc(c->N); t_vel.alloc(c->N); t_quat.alloc(c->N); t_ang.alloc(c->N); t_last.alloc(c->N);
-    t_bd.alloc(c->N); t_bdtq.alloc(c->N); t_f.alloc(c->N); t_t.alloc(c->N);
-    const RunState keep{t_pos.p, t_vel.p, t_quat.p, t_ang.p, t_last.p, t_bd.p, t_bdtq.p, t_f.p, t_t.p};
+    // the starting state, kept apart from the run's own per-chunk save (released however
+    // the tuner ends)
+    struct Kept {
+        DBuf<double4> pos, vel, quat, ang, last, bd, bdtq;
+        DBuf<float4> f, t;
+        ~Kept() {
+            pos.release(); vel.release(); quat.release(); ang.release(); last.release();
+            bd.release(); bdtq.release(); f.release(); t.release();
+        }
+    } kept;
+    kept.pos.alloc(c->N); kept.vel.alloc(c->N); kept.quat.alloc(c->N); kept.ang.alloc(c->N);
+    kept.last.alloc(c->N); kept.bd.alloc(c->N); kept.bdtq.alloc(c->N); kept.f.alloc(c->N); kept.t.alloc(c->N);
+    const RunState keep{kept.pos.p, kept.vel.p, kept.quat.p, kept.ang.p, kept.last.p,
+                        kept.bd.p, kept.bdtq.p, kept.f.p, kept.t.p};
     hipLaunchKernelGGL(k_copy_state, dim3(grid_stride_blocks(c->N)), dim3(kBlock), 0, c->stream, c->N,
                        live_state(c), keep);
     HIP_OK(hipGetLastError());
     const uint64_t step_kept = c->host_step;
+    const double buffer_kept = c->buffer;
     DevFlags f0;
     read_flags(c, f0);
     auto set_skin = [&](double b) {
@@ -1910,31 +1919,42 @@ double tune_buffer(oxdna_ctx* c, uint32_t mask, const std::vector<double>& cand,
         c->tuned_mask = 0xffffffffu;
         compute(c, m, 1);
     };
+    // the step and build counts as they stood (the trials' steps are not the caller's)
+    auto restore_counts = [&]() {
+        DevFlags f;
+        read_flags(c, f);
+        f.step = f0.step;
+        f.builds = f0.builds;
+        HIP_OK(hipMemcpyAsync(c->flags.p, &f, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    };
     double best = -1.0, best_b = cand[0];
-    for (size_t k = 0; k < cand.size(); ++k) {
-        set_skin(cand[k]);
-        if (warm) run(c, warm, m, 1);
-        HIP_OK(hipStreamSynchronize(c->stream));
-        const auto t0 = std::chrono::steady_clock::now();
-        run(c, steps, m, 1);
-        HIP_OK(hipStreamSynchronize(c->stream));
-        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        const double rate = steps / std::max(sec, 1e-9);
-        if (rates) rates[k] = rate;
-        if (rate > best) { best = rate; best_b = cand[k]; }
+    try {
+        for (size_t k = 0; k < cand.size(); ++k) {
+            set_skin(cand[k]);
+            if (warm) run(c, warm, m, 1);
+            HIP_OK(hipStreamSynchronize(c->stream));
+            const auto t0 = std::chrono::steady_clock::now();
+            run(c, steps, m, 1);
+            HIP_OK(hipStreamSynchronize(c->stream));
+            const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            const double rate = steps / std::max(sec, 1e-9);
+            if (rates) rates[k] = rate;
+            if (rate > best) { best = rate; best_b = cand[k]; }
+        }
+    } catch (...) {
+        // a failed trial leaves the caller's state, skin and counts as they were
+        try {
+            set_skin(buffer_kept);
+            restore_counts();
+        } catch (...) {
+        }
+        throw;
     }
     set_skin(best_b);
     autotune(c, m);
     build_graph(c, m);  // the caller's next run replays it instead of capturing it
-    // the step and build counts as they stood (the trials' steps are not the caller's)
-    DevFlags f;
-    read_flags(c, f);
-    f.step = f0.step;
-    f.builds = f0.builds;
-    HIP_OK(hipMemcpyAsync(c->flags.p, &f, sizeof(DevFlags), hipMemcpyHostToDevice, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
-    t_pos.release(); t_vel.release(); t_quat.release(); t_ang.release(); t_last.release();
-    t_bd.release(); t_bdtq.release(); t_f.release(); t_t.release();
+    restore_counts();
     return best_b;
 }
 
