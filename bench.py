@@ -292,6 +292,14 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    # torch (plumbing only) is initialised first: its set-up takes seconds of host time,
+    # which must not fall between the warm-up steps and the timed ones
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+        sync()
+    except Exception:  # the engine stream sync is what matters
+        sync = lambda: None  # noqa: E731
     import numpy as np
     from oxdna import sim, systems
     from oxdna.model import dna1, dna2
@@ -304,16 +312,12 @@ def main():
     s.create_state_from_snapshot(system.snapshot())
     s.operations.integrator = sim.Integrator(
         dt=0.003, forces=forces, methods=[sim.Langevin(kT=dna1.kT_from_temperature(296.15))])
-    s.run(max(args.warmup, 1))
-    # the neighbour-list skin from the autotuner (buffers 0.1 ... 0.8, 2000 timed steps
-    # each from the warmed-up state, which it restores), unless --buffer pins it
+    # the neighbour-list skin from the autotuner (buffers 0.1 ... 0.8, 400 untimed and
+    # 2000 timed steps each from the initial state, which it restores), unless --buffer
+    # pins it; then the warm-up steps, right before the timed ones
+    s.run(0)
     skin_rates = None if args.buffer > 0 else s.tune_nlist_buffer(steps=2000)
-    s.synchronize()
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is plumbing only; the engine stream sync is what matters
-        sync = lambda: None  # noqa: E731
+    s.run(max(args.warmup, 1))
     sync(); s.synchronize()
     t0 = time.perf_counter()
     s.run(args.steps)
