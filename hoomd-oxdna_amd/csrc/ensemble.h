@@ -15,7 +15,8 @@
 //           bit words per 32 rows (near, far) of particle i;
 //   scan    each particle's near / far counts are scanned in particle order: near
 //           entries fill the pair records from the front, far ones from the back, each
-//           particle's entries together in d order (its list is written here);
+//           particle's entries together in d order; with one pass, every thread then
+//           writes the list for one (particle, 8 rows) item at a time;
 //   eval    whole wavefronts evaluate near entries, then far ones (Debye-Hueckel only),
 //           then the bonds and the bias; a pair's record holds the force and torque on
 //           its first end and the torque on its second (F_j = -F_i, and
@@ -288,9 +289,9 @@ __global__ __launch_bounds__(NT) void k_ensemble(EnsArgs a) {
 #pragma unroll
     for (int s = 0; s < 14; ++s) cnt[s] = 0u;
 
-    // List entries and incoming-record entries of particle p (lanes of its group): p's
-    // near entries start at record en, its far ones at cap - 1 - ef going down; lane q
-    // writes those of rows 32 Wn q / G + 1 .. 32 Wn (q + 1) / G.
+    // List entries and incoming-record entries of particle p in a pass of several (lanes
+    // of its group): p's near entries start at record en, its far ones at cap - 1 - ef
+    // going down; lane q writes those of rows 32 Wn q / G + 1 .. 32 Wn (q + 1) / G.
     auto fill = [&](uint32_t en, uint32_t ef) {
         const uint32_t chunk = 32u * Wn / G, lo = q * chunk, hi = lo + chunk;
         for (uint32_t w = 0; w < Wn; ++w) {
@@ -420,16 +421,14 @@ __global__ __launch_bounds__(NT) void k_ensemble(EnsArgs a) {
             const uint32_t gf = pf + wave_incl_scan(cf) - cf;
             uint32_t mypass = 0u;
             if (tn + tf <= cap) {
-                // one pass: the offsets are the scan; the list is written here
+                // one pass: the offsets are the scan; the list is written after the
+                // barrier, by every thread
                 if (t == 0) {
                     Sctl[0] = 1u;
                     Spass[0] = make_uint2(0u, 0u);
                     Spass[1] = make_uint2(tn, tf);
                 }
-                const uint32_t gnp = __shfl(gn, (int)(lane & ~(G - 1u)));  // the leader's offsets
-                const uint32_t gfp = __shfl(gf, (int)(lane & ~(G - 1u)));
                 if (leader) Spt[p] = make_uint2(gn | (gf << 16), 0u);
-                if (grp) fill(gnp, gfp);
             } else {
                 // passes over ranges of first ends: particle p goes to pass
                 // (its combined prefix) / H; a particle holds at most D entries, so no
@@ -459,6 +458,42 @@ __global__ __launch_bounds__(NT) void k_ensemble(EnsArgs a) {
         }
         __syncthreads();
         const uint32_t npass = Sctl[0];
+        if (npass == 1) {
+            // ---- fill (one pass): list entries and incoming-record entries, one item
+            //      (particle i, rows 8 b + 1 .. 8 b + 8) per thread in turn; an entry's
+            //      record is i's offset plus the entries of i's earlier rows, the order
+            //      of the per-particle fill below
+            for (uint32_t u = t; u < items; u += NT) {
+                const uint32_t i = u % n, b = u / n, w = b >> 2, sh = 8u * (b & 3u);
+                const uint2 x = Sbits[w * ncap + i];
+                uint32_t nb = (x.x >> sh) & 0xffu, fb = (x.y >> sh) & 0xffu;
+                if (!(nb | fb)) continue;
+                const uint2 pt = Spt[i];
+                uint32_t en = pt.x & 0xffffu, ef = pt.x >> 16;
+                for (uint32_t w2 = 0; w2 < w; ++w2) {
+                    const uint2 y = Sbits[w2 * ncap + i];
+                    en += __popc(y.x);
+                    ef += __popc(y.y);
+                }
+                const uint32_t below = (1u << sh) - 1u;
+                en += __popc(x.x & below);
+                ef += __popc(x.y & below);
+                while (nb) {
+                    const uint32_t d = 8u * b + (uint32_t)__ffs(nb);
+                    nb &= nb - 1u;
+                    if (has_inc) Sinc[(d - 1u) * ncap + (i + d < n ? i + d : i + d - n)] = (uint16_t)en;
+                    Rt[en++].w = __uint_as_float(i | (d << 16));
+                }
+                while (fb) {
+                    const uint32_t d = 8u * b + (uint32_t)__ffs(fb);
+                    fb &= fb - 1u;
+                    const uint32_t e = cap - 1u - ef++;
+                    if (has_inc) Sinc[(d - 1u) * ncap + (i + d < n ? i + d : i + d - n)] = (uint16_t)e;
+                    Rt[e].w = __uint_as_float(i | (d << 16));
+                }
+            }
+            __syncthreads();
+        }
 
         for (uint32_t pass = 0; pass < npass; ++pass) {
             const uint2 pb = Spass[pass], pe = Spass[pass + 1];
