@@ -1058,6 +1058,8 @@ struct oxdna_ctx {
     bool have_bias = false;
     int ens_nt = 0;                      // workgroup size of the replica kernel (autotuned)
     uint32_t ens_div = 1;                // its tile depth: the LDS share's record capacity / ens_div (autotuned)
+    int fixed_ens_nt = 0;                // nonzero: replica workgroup size pinned by the caller ...
+    uint32_t fixed_ens_div = 1;          // ... with this tile-depth divisor
     uint32_t ens_tuned_mask = 0xffffffffu;
 
     ~oxdna_ctx() {
@@ -2095,6 +2097,15 @@ void autotune_ens(oxdna_ctx* c, uint32_t m) {
         return;
     }
     if (c->ens_tuned_mask == m && c->ens_nt != 0) return;
+    if (c->fixed_ens_nt) {
+        if (c->fixed_ens_nt < ens_nmax(c))
+            throw Fail("oxdna_set_replica_tiling: a workgroup of " + std::to_string(c->fixed_ens_nt) +
+                       " threads is smaller than the largest replica needs (" + std::to_string(ens_nmax(c)) + ")");
+        c->ens_nt = c->fixed_ens_nt;
+        c->ens_div = c->fixed_ens_div;
+        c->ens_tuned_mask = m;
+        return;
+    }
     float best = 1e30f;
     int best_nt = ens_default_nt(c);
     uint32_t best_div = 1;
@@ -2131,6 +2142,7 @@ void autotune_ens(oxdna_ctx* c, uint32_t m) {
 void compute_ens(oxdna_ctx* c, uint32_t mask) {
     uint32_t m = effective_mask(c, mask);
     ensure_ready_ens(c, m);
+    if (c->fixed_ens_nt) autotune_ens(c, m);  // takes the pinned shape, times nothing
     int nt = c->P.xwords ? 512 : (c->ens_nt ? c->ens_nt : ens_default_nt(c));
     launch_ens(c, nt, false, ens_args(c, m, 0, 1, 1));
     hipLaunchKernelGGL(k_rep_sum, dim3(1), dim3(256), 0, c->stream, c->n_rep, c->rep_energy.p, c->energy.p);
@@ -2760,6 +2772,22 @@ int oxdna_set_lanes_per_particle(oxdna_ctx* c, int lanes) {
         c->fixed_tpp = lanes;
         if (lanes) c->tpp = lanes;
         c->tuned_mask = 0xffffffffu;
+        c->drop_graph();
+    });
+}
+
+int oxdna_set_replica_tiling(oxdna_ctx* c, int workgroup, int depth_divisor) {
+    if (!c) return fail(nullptr, "NULL context");
+    return guard(c, [&] {
+        if (workgroup != 0 && workgroup != 64 && workgroup != 128 && workgroup != 256 && workgroup != 512)
+            throw Fail("oxdna_set_replica_tiling: workgroup 0 (autotune), 64, 128, 256 or 512");
+        if (workgroup != 0 && depth_divisor != 1 && depth_divisor != 2 && depth_divisor != 4)
+            throw Fail("oxdna_set_replica_tiling: depth_divisor 1, 2 or 4");
+        c->fixed_ens_nt = workgroup;
+        c->fixed_ens_div = workgroup ? (uint32_t)depth_divisor : 1u;
+        c->ens_nt = 0;
+        c->ens_div = 1;
+        c->ens_tuned_mask = 0xffffffffu;
         c->drop_graph();
     });
 }

@@ -22,14 +22,17 @@
 //           its first end and the torque on its second (F_j = -F_i, and
 //           tau_j = -tau_i + (r_j - r_i) x F_i for a rotation-invariant pair energy);
 //   gather  particle t sums its first-end records (contiguous), then its second-end
-//           records (found from the bit words of particle t - d and its scan offset),
-//           its bond records and the bias, always in the same order, so a run is
-//           bit-for-bit repeatable; then it integrates.
+//           records (for replicas of up to 128 particles from a table of incoming
+//           record indices that the fill writes, else from the bit words of particle
+//           t - d and its scan offset), its bond records and the bias, always in the
+//           same order, so a run is bit-for-bit repeatable; then it integrates.
 //
 // No atomics: every sum has a fixed order. When a step's entries exceed the record
 // capacity the pairs are split into passes over ranges of first ends; each pass fills,
 // evaluates and gathers its own records.
 #pragma once
+
+#include <type_traits>
 
 #include "integrator.h"
 #include "kernels.h"
@@ -344,8 +347,10 @@ __global__ __launch_bounds__(NT) void k_ensemble(EnsArgs a) {
                     const bool valid = d <= D && !(2u * d == n && i >= d);
                     uint32_t j = i + d;
                     if (j >= n) j -= n;
+                    // every entry of rows 1 .. D is reset, the never-filled second ends
+                    // p < n / 2 of row n / 2 included, so the gather reads them unguarded
+                    if (d <= D && has_inc) Sinc[(d - 1u) * ncap + j] = 0xffffu;
                     if (!valid) j = i;
-                    if (valid && has_inc) Sinc[(d - 1u) * ncap + j] = 0xffffu;
                     jv[k] = j;
                     ok[k] = valid && (int)j != bo.x && (int)j != bo.y;
                 }
@@ -592,7 +597,7 @@ __global__ __launch_bounds__(NT) void k_ensemble(EnsArgs a) {
 
             // ---- gather: lane q of particle p's group adds p's first-end records
             //      q, q + G, ..., then its second-end records of rows d = q + 1, q + 1 + G,
-            //      ..., four at a time (a missing record reads the zero record); the group
+            //      ..., eight at a time (a missing record reads the zero record); the group
             //      sums its lanes; the leader adds the bonds and the bias (first pass) and
             //      the sums of earlier passes
             V F = zero3(), T = zero3();
@@ -612,42 +617,65 @@ __global__ __launch_bounds__(NT) void k_ensemble(EnsArgs a) {
                         cf += __popc(x.y);
                     }
                     const uint32_t on = pt.x & 0xffffu, of = pt.x >> 16;
-                    for (uint32_t k = q; k < cn + cf; k += 4u * G) {
-                        uint32_t e[4];
+                    for (uint32_t k = q; k < cn + cf; k += 8u * G) {
+                        uint32_t e[8];
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) {
+                        for (int c = 0; c < 8; ++c) {
                             const uint32_t kk = k + c * G;
                             e[c] = kk < cn ? on + kk : (kk < cn + cf ? cap - 1u - (of + kk - cn) : cap);
                         }
+                        float4 f[8], m[8];
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) {
-                            const float4 f = Rf[e[c]], m = Rt[e[c]];
-                            F += mk(f.x, f.y, f.z);
-                            T += mk(m.x, m.y, m.z);
-                            E += 0.5f * f.w;
+                        for (int c = 0; c < 8; ++c) { f[c] = Rf[e[c]]; m[c] = Rt[e[c]]; }
+                        // all sixteen reads in flight before the first sum
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int c = 0; c < 8; ++c) {
+                            F += mk(f[c].x, f[c].y, f[c].z);
+                            T += mk(m[c].x, m[c].y, m[c].z);
+                            E += 0.5f * f[c].w;
                         }
                     }
                 }
-                if (has_inc && npass == 1) {
-                    for (uint32_t d0 = 1u + q; d0 <= D; d0 += 4u * G) {
-                        uint32_t e[4];
+                // an entry is this pass's record when its first end p - d is in this pass
+                // (entries of earlier passes are stale, of later ones unset); one pass
+                // skips that test and its reads
+                auto second_ends = [&](auto several) {
+                    for (uint32_t d0 = 1u + q; d0 <= D; d0 += 8u * G) {
+                        uint32_t e[8], ps[8];
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) {
-                            const uint32_t d = d0 + c * G;
-                            uint32_t v = 0xffffu;
-                            if (d <= D && !(2u * d == n && p < d)) v = Sinc[(d - 1u) * ncap + p];
-                            e[c] = v == 0xffffu ? cap : v;
+                        for (int c = 0; c < 8; ++c) {
+                            const uint32_t d = min(d0 + c * G, D);
+                            e[c] = Sinc[(d - 1u) * ncap + p];
+                            if constexpr (decltype(several)::value) ps[c] = Spt[p >= d ? p - d : p + n - d].y;
                         }
+                        // the compiler would wait for each read before issuing the next
+                        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) {
-                            const float4 f = Rf[e[c]], m = Rj[e[c]];
-                            F -= mk(f.x, f.y, f.z);
-                            T += mk(m.x, m.y, m.z);
-                            E += 0.5f * f.w;
+                        for (int c = 0; c < 8; ++c) {
+                            bool none = d0 + c * G > D || e[c] == 0xffffu;
+                            if constexpr (decltype(several)::value) none = none || ps[c] != pass;
+                            if (none) e[c] = cap;
+                        }
+                        float4 f[8], m[8];
+#pragma unroll
+                        for (int c = 0; c < 8; ++c) { f[c] = Rf[e[c]]; m[c] = Rj[e[c]]; }
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int c = 0; c < 8; ++c) {
+                            F -= mk(f[c].x, f[c].y, f[c].z);
+                            T += mk(m[c].x, m[c].y, m[c].z);
+                            E += 0.5f * f[c].w;
                         }
                     }
+                };
+                if (has_inc && npass == 1) {
+                    second_ends(std::false_type{});
+                } else if (has_inc) {
+                    second_ends(std::true_type{});
                 } else {
-                    // second-end records found from the bit words of particle p - d
+                    // (replicas above the table's size) second-end records found from the
+                    // bit words of particle p - d
                     for (uint32_t d = 1u + q; d <= D; d += G) {
                         uint32_t i = p + n - d;
                         if (i >= n) i -= n;

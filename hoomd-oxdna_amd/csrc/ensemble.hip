@@ -3,7 +3,7 @@
 // loop-invariant code motion: with it, the constants and invariant products of the
 // integrator and of the pair terms are hoisted out of the step loop, both sets stay live
 // across the whole loop, and the kernel needs more than 256 VGPRs, one wavefront per
-// SIMD; without it, 254).
+// SIMD; without it, 243).
 #include "ensemble.h"
 
 namespace oxd {

@@ -41,6 +41,7 @@ _SIGS = {
     "oxdna_synchronize": (C.c_int, [_p]),
     "oxdna_get_stats": (C.c_int, [_p, C.POINTER(_u64), C.POINTER(_u32), C.POINTER(_u64)]),
     "oxdna_set_lanes_per_particle": (C.c_int, [_p, C.c_int]),
+    "oxdna_set_replica_tiling": (C.c_int, [_p, C.c_int, C.c_int]),
     "oxdna_get_tuning": (C.c_int, [_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "oxdna_get_tuning_ex": (C.c_int, [_p, C.POINTER(_abi.Tuning)]),
     "oxdna_tune_nlist_buffer": (C.c_int, [_p, _u32, _dp, _u32, _u32, _dp, _dp]),

@@ -110,6 +110,16 @@ class ReplicaEnsemble(sim.Simulation):
         snap, self._offsets, self._boxes = concatenate_snapshots(list(snaps))
         self.create_state_from_snapshot(snap)
 
+    def set_tiling(self, workgroup=0, depth_divisor=1):
+        """Pins the replica kernel's shape instead of autotuning it: `workgroup` threads
+        per replica (64, 128, 256 or 512, at least the largest replica) and pair records
+        per pass of the LDS share's capacity / `depth_divisor` (1, 2 or 4). workgroup=0
+        hands both back to the autotuner. Every shape gives the same results up to FP32
+        summation order."""
+        self._check(self._lib.oxdna_set_replica_tiling(self._ctx, int(workgroup), int(depth_divisor)),
+                    "set_replica_tiling")
+        self._cache.clear()
+
     def _upload_state(self):
         super()._upload_state()
         if self._offsets is None:

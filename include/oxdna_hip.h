@@ -353,6 +353,16 @@ int oxdna_get_stats(oxdna_ctx* ctx, uint64_t* n_builds, uint32_t* max_neigh, uin
  * reruns are bit-identical only with the same value: pin it for bit-reproducible
  * trajectories across processes. */
 int oxdna_set_lanes_per_particle(oxdna_ctx* ctx, int lanes);
+/* Replica kernel shape, pinned: workgroup of 64, 128, 256 or 512 threads (at least the
+ * largest replica's size rounded up to a power of two, checked at the next run) and a
+ * tile depth of the LDS share's pair-record capacity / depth_divisor (1, 2 or 4; at
+ * least the largest replica + 64 records): a shallower tile runs more passes over
+ * ranges of first ends per step and leaves LDS for more workgroups per compute unit.
+ * workgroup 0 returns both to the autotuner. Replica sets with per-type-pair parameter
+ * tables always run 512-thread workgroups at full depth. Plays the part of setting
+ * HOOMD's AutotunedObject.kernel_parameters, the interface of the Autotuner the
+ * reference registers its kernels with (src/OxDNABondedForceComputeGPU.cc:35-37). */
+int oxdna_set_replica_tiling(oxdna_ctx* ctx, int workgroup, int depth_divisor);
 /* Autotuner choices: lanes per particle of the force kernel (4, 8, 16 or 32) and the
  * workgroup size of the replica kernel (64..512; 0 before the first replica run). */
 int oxdna_get_tuning(oxdna_ctx* ctx, int* lanes_per_particle, int* replica_workgroup);

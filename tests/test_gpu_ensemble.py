@@ -38,8 +38,10 @@ def _ensemble(reps, forces, seed=3, langevin=False):
     return ens
 
 
-def _check_replicas(reps, forces, label, bias=None):
+def _check_replicas(reps, forces, label, bias=None, tiling=None):
     ens = _ensemble(reps, forces)
+    if tiling is not None:
+        ens.set_tiling(*tiling)
     if bias is not None:
         ens.bias = bias
     ens.run(0)
@@ -127,6 +129,72 @@ def test_uneven_sizes_one_launch_up_to_496():
     ens2 = ensemble.ReplicaEnsemble(device=sim.GPU(), seed=3)
     with pytest.raises(Exception, match="at most 496"):
         ens2.create_state_from_snapshots([over.snapshot()])
+
+
+def _records(rep, forces):
+    """Pair records of a replica: pairs whose FP64 minimum-image centre distance lies
+    inside the largest cut-off of any term (the kernel's test)."""
+    rmax = max(max(f._pack(["A", "G", "C", "T"])[1]) for f in forces[1:])
+    d = rep.pos[:, None] - rep.pos[None]
+    d -= rep.box * np.rint(d / rep.box)
+    r2 = (d * d).sum(-1)[np.triu_indices(rep.n, 1)]
+    return int((r2 < rmax * rmax).sum())
+
+
+@pytest.mark.parametrize("tiling", [(128, 4), (256, 2), (512, 1)])
+def test_pinned_tilings_match_oracle(tiling):
+    """Every pinned replica-kernel shape gives the oracle's forces: (128, 4) holds fewer
+    records per pass than the largest hairpin has, so its pairs run in several passes
+    and the second ends are found through the incoming-record table with the pass
+    check; the others run the same replicas in one pass."""
+    reps = [H.jiggle(h, seed=60 + k, dx=0.03, dq=0.05) for k, h in enumerate(systems.hairpin_ensemble(12, seed=3))]
+    forces, _ = dna2.forces(fused=True)
+    ens = _check_replicas(reps, forces, f"hairpins tiling {tiling}", tiling=tiling)
+    t = ens.tuning()
+    most = max(_records(r, forces) for r in reps)
+    _say(f"tiling {tiling}: {t['replica_records_per_pass']} records per pass, largest replica {most} records")
+    assert t["replica_workgroup"] == tiling[0]
+    if tiling == (128, 4):
+        assert t["replica_records_per_pass"] < most, "the pinned depth should force several passes"
+
+
+def test_multipass_small_replicas_are_bit_identical_and_follow_one_pass():
+    """With several passes per step on replicas small enough for the incoming-record
+    table, two runs are bit-identical, and 100 Langevin steps follow the one-pass shape
+    up to FP32 summation order."""
+    reps = list(systems.hairpin_ensemble(16, seed=23))
+    forces, _ = dna2.forces(fused=True)
+    out = []
+    for tiling in ((128, 4), (128, 4), (512, 1)):
+        ens = _ensemble(reps, forces, seed=9, langevin=True)
+        ens.set_tiling(*tiling)
+        ens.run(100)
+        snap = ens.state.get_snapshot()
+        out.append((snap.particles.position.copy(), snap.particles.velocity.copy()))
+        del ens
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    dx = np.abs(out[0][0] - out[2][0]).max()
+    dv = np.abs(out[0][1] - out[2][1]).max()
+    _say(f"multi-pass vs one-pass after 100 steps: |dx| {dx:.2e}, |dv| {dv:.2e}")
+    assert dx < 1e-3 and dv < 1e-2
+
+
+def test_pinned_tiling_rejects_bad_shapes():
+    reps = list(systems.hairpin_ensemble(4, seed=2))
+    forces, _ = dna2.forces(fused=True)
+    ens = _ensemble(reps, forces)
+    with pytest.raises(Exception, match="workgroup"):
+        ens.set_tiling(96, 1)
+    with pytest.raises(Exception, match="depth_divisor"):
+        ens.set_tiling(128, 3)
+    big = systems.duplex_gas(4, 20, box=40.0, seed=4, min_dist=1.5)  # 160 nt
+    ens2 = _ensemble([big], forces, langevin=True)
+    ens2.set_tiling(128, 1)
+    with pytest.raises(Exception, match="smaller than the largest replica"):
+        ens2.run(0)
+    ens2.set_tiling(0)  # back to the autotuner, which times the shapes at the first run
+    ens2.run(1)
+    assert ens2.tuning()["replica_workgroup"] >= 256
 
 
 def test_replicas_follow_one_system_runs():
