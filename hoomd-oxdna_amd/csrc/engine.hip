@@ -2050,37 +2050,51 @@ struct EnsShape {
     uint32_t ncap, cap, lds;
 };
 
+uint32_t ens_ncap(const oxdna_ctx* c) { return std::max<uint32_t>(8u, (c->rep_nmax + 7u) & ~7u); }
+
+// The general replica instantiation (512 threads, full depth, the parameter block read in
+// global memory) runs when the tables differ between type pairs or when the largest
+// replica's layout with the block in LDS does not fit one workgroup at the minimum depth
+// (489-496 particles); every other launch keeps the block in LDS.
+bool ens_general(const oxdna_ctx* c) {
+    if (c->P.xwords) return true;
+    const uint32_t ncap = ens_ncap(c);
+    return ens_layout(ncap, ncap + 64u, 512u, true).bytes > c->lds_block;
+}
+
 EnsShape ens_shape(const oxdna_ctx* c, int nt, uint32_t div = 1) {
     EnsShape s{};
-    s.ncap = std::max<uint32_t>(8u, (c->rep_nmax + 7u) & ~7u);
+    s.ncap = ens_ncap(c);
+    const bool staged = !ens_general(c);
     const uint32_t ncu = std::max<uint32_t>(1u, c->ncu);
     const uint32_t waves = (uint32_t)nt / 64u;
     const uint32_t need = s.ncap + 64u;
     const uint32_t pairs = s.ncap * (s.ncap - 1u) / 2u;
-    const uint32_t base = ens_layout(s.ncap, 0u, (uint32_t)nt).bytes;
+    const uint32_t base = ens_layout(s.ncap, 0u, (uint32_t)nt, staged).bytes;
     uint32_t k = std::min<uint32_t>((c->n_rep + ncu - 1u) / ncu, std::max<uint32_t>(1u, 8u / waves));
     for (k = std::max<uint32_t>(k, 1u); k >= 1u; --k) {
         const uint32_t budget = std::min<uint32_t>(c->lds_block, c->lds_cu / k);
         if (base + 48u * need > budget) continue;
         s.cap = std::min<uint32_t>((budget - base) / 48u, std::max(need, pairs));
         s.cap = std::max(need, s.cap / std::max<uint32_t>(div, 1u));
-        s.lds = ens_layout(s.ncap, s.cap, (uint32_t)nt).bytes;
+        s.lds = ens_layout(s.ncap, s.cap, (uint32_t)nt, staged).bytes;
         return s;
     }
     throw Fail("replicas of " + std::to_string(c->rep_nmax) + " particles do not fit the LDS of one workgroup (" +
                std::to_string(c->lds_block) + " bytes)");
 }
 
-// Launches the replica kernel with workgroup size nt (count: the counting variant). With
-// per-type-pair parameter sets, and for the count, the 512-thread instantiation runs.
+// Launches the replica kernel with workgroup size nt (count: the counting variant). The
+// general instantiation, and the count, run 512-thread workgroups at full depth.
 void launch_ens(oxdna_ctx* c, int nt, bool count, EnsArgs a) {
-    if (c->P.xwords || count) nt = 512;
+    const bool general = ens_general(c);
+    if (general || count) nt = 512;
     if (nt < ens_nmax(c) || a.n_rep == 0 || !a.offsets || !a.order || !a.P || c->rep_nmax > (uint32_t)nt)
         throw Fail("internal: replica kernel launch does not fit the replicas");
-    const EnsShape sh = ens_shape(c, nt, (c->P.xwords || count) ? 1u : c->ens_div);
+    const EnsShape sh = ens_shape(c, nt, (general || count) ? 1u : c->ens_div);
     a.ncap = sh.ncap;
     a.cap = sh.cap;
-    HIP_OK(launch_ensemble(nt, count, c->P.xwords != 0, a.n_rep, sh.lds, c->stream, a));
+    HIP_OK(launch_ensemble(nt, count, general, a.n_rep, sh.lds, c->stream, a));
 }
 
 int ens_default_nt(const oxdna_ctx* c) { return std::max(256, ens_nmax(c)); }
@@ -2090,7 +2104,7 @@ int ens_default_nt(const oxdna_ctx* c) { return std::max(256, ens_nmax(c)); }
 // share's full depth, a half or a quarter of it, each distinct depth once), timed as
 // forces-only launches (the state is not changed) with HIP events; keeps the fastest.
 void autotune_ens(oxdna_ctx* c, uint32_t m) {
-    if (c->P.xwords) {  // the general instantiation has one workgroup size
+    if (ens_general(c)) {  // the general instantiation has one workgroup size
         c->ens_nt = 512;
         c->ens_div = 1;
         c->ens_tuned_mask = m;
@@ -2143,7 +2157,7 @@ void compute_ens(oxdna_ctx* c, uint32_t mask) {
     uint32_t m = effective_mask(c, mask);
     ensure_ready_ens(c, m);
     if (c->fixed_ens_nt) autotune_ens(c, m);  // takes the pinned shape, times nothing
-    int nt = c->P.xwords ? 512 : (c->ens_nt ? c->ens_nt : ens_default_nt(c));
+    int nt = ens_general(c) ? 512 : (c->ens_nt ? c->ens_nt : ens_default_nt(c));
     launch_ens(c, nt, false, ens_args(c, m, 0, 1, 1));
     hipLaunchKernelGGL(k_rep_sum, dim3(1), dim3(256), 0, c->stream, c->n_rep, c->rep_energy.p, c->energy.p);
     HIP_OK(hipMemsetAsync(c->virial.p, 0, sizeof(float) * 6 * (size_t)c->N, c->stream));

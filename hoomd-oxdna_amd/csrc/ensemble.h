@@ -89,14 +89,15 @@ __host__ __device__ inline uint32_t ens_words(uint32_t ncap) {
 
 // Byte offsets of the dynamic LDS arrays of one replica (ncap particles, cap records).
 struct EnsLayout {
-    uint32_t xy, z, f0, f1, f2, bond, brf, brt, bits, pt, gl, inc, acc, pass, red, ctl, rf, rt, rj, bytes;
+    uint32_t par, xy, z, f0, f1, f2, bond, brf, brt, bits, pt, gl, inc, acc, pass, red, ctl, rf, rt, rj, bytes;
 };
 
 // Replicas of up to this many particles keep a table of incoming record indices (one
 // 16-bit entry per pair, 16 KB at 128); larger ones find them from the bit words.
 constexpr uint32_t kEnsIncMax = 128;
 
-__host__ __device__ inline EnsLayout ens_layout(uint32_t ncap, uint32_t cap, uint32_t nt) {
+// staged: the parameter block (DParams) is copied to the start of the layout.
+__host__ __device__ inline EnsLayout ens_layout(uint32_t ncap, uint32_t cap, uint32_t nt, bool staged) {
     EnsLayout l{};
     uint32_t o = 0;
     auto take = [&o](uint32_t bytes) {
@@ -104,6 +105,7 @@ __host__ __device__ inline EnsLayout ens_layout(uint32_t ncap, uint32_t cap, uin
         o += (bytes + 15u) & ~15u;
         return at;
     };
+    l.par = take(staged ? (uint32_t)sizeof(DParams) : 0u);  // DParams, read by every pair
     l.xy = take(16 * ncap);                     // double2 x, y
     l.z = take(8 * ncap);                       // double z
     l.f0 = take(16 * ncap);                     // float4 a1 + charge
@@ -171,8 +173,10 @@ __device__ __forceinline__ float group_sum(float v, uint32_t G) {
     return v;
 }
 
-// MULTI: some table differs between type pairs (each pair finds its sets through their
-// offsets in the parameter block; otherwise the base sets, at fixed offsets).
+// MULTI: the general instantiation, for tables that differ between type pairs and for
+// replicas too large for a layout with the parameter block: it reads the block in global
+// memory, each pair finding its sets through their offsets in it. Otherwise the block is
+// copied to LDS and the base sets sit at fixed offsets.
 template<int NT, bool COUNT, bool MULTI>
 __global__ __launch_bounds__(NT) void k_ensemble(EnsArgs a) {
     static_assert(NT % 64 == 0 && NT <= 512, "whole wavefronts");
@@ -185,7 +189,7 @@ __global__ __launch_bounds__(NT) void k_ensemble(EnsArgs a) {
     extern __shared__ uint4 ens_lds[];
     unsigned char* lds = reinterpret_cast<unsigned char*>(ens_lds);
     const uint32_t ncap = a.ncap, cap = a.cap;
-    const EnsLayout lo = ens_layout(ncap, cap, NT);
+    const EnsLayout lo = ens_layout(ncap, cap, NT, !MULTI);
     double2* Sxy = reinterpret_cast<double2*>(lds + lo.xy);
     double* Sz = reinterpret_cast<double*>(lds + lo.z);
     float4* Sf0 = reinterpret_cast<float4*>(lds + lo.f0);
@@ -228,7 +232,16 @@ __global__ __launch_bounds__(NT) void k_ensemble(EnsArgs a) {
     const bool leader = grp && q == 0;
     const double L[3] = {a.boxes[3 * r], a.boxes[3 * r + 1], a.boxes[3 * r + 2]};
     const double Linv[3] = {1.0 / L[0], 1.0 / L[1], 1.0 / L[2]};
-    const DParams& P = *a.P;
+    // uniform tables: the parameter block in LDS, so a cut-off or parameter read is an
+    // LDS read, not a global round trip in every branch of the pair terms; MULTI reads
+    // it (and the per-type-pair sets after it) in global memory
+    if constexpr (!MULTI) {
+        uint4* Spar = reinterpret_cast<uint4*>(lds + lo.par);
+        for (uint32_t w = t; w < (uint32_t)(sizeof(DParams) / 16); w += NT)
+            Spar[w] = reinterpret_cast<const uint4*>(a.P)[w];
+        __syncthreads();
+    }
+    const DParams& P = MULTI ? *a.P : *reinterpret_cast<const DParams*>(lds + lo.par);
     const uint32_t mask = a.mask;
     const bool bonded = (mask & (1u << T_BONDED)) != 0u;
     const int nt = P.ntypes;

@@ -359,7 +359,8 @@ int oxdna_set_lanes_per_particle(oxdna_ctx* ctx, int lanes);
  * least the largest replica + 64 records): a shallower tile runs more passes over
  * ranges of first ends per step and leaves LDS for more workgroups per compute unit.
  * workgroup 0 returns both to the autotuner. Replica sets with per-type-pair parameter
- * tables always run 512-thread workgroups at full depth. Plays the part of setting
+ * tables, and replicas above 488 particles, always run 512-thread workgroups at full
+ * depth. Plays the part of setting
  * HOOMD's AutotunedObject.kernel_parameters, the interface of the Autotuner the
  * reference registers its kernels with (src/OxDNABondedForceComputeGPU.cc:35-37). */
 int oxdna_set_replica_tiling(oxdna_ctx* ctx, int workgroup, int depth_divisor);
