@@ -5,11 +5,14 @@ A kernel whose code never reads the grid or workgroup size gets none, so its arg
 segment ends at its explicit arguments; a tool that reads a dispatch's hidden arguments
 (a profiler's dispatch record) then reads past the end of the argument buffer. The
 replica kernel was such a kernel; it now checks its grid against the replica count.
-Needs objcopy and the ROCm LLVM tools (present in this image); built-library code only.
+Needs the ROCm LLVM tools (present in this image); built-library code only. The library
+is only read: its fat-binary section is taken from the ELF section table here (an
+objcopy run without an output file rewrites its input in place, which corrupts the
+mapping of a process that has the library loaded).
 """
 
 import os
-import shutil
+import struct
 import subprocess
 
 import pytest
@@ -23,20 +26,34 @@ LLVM = "/opt/rocm/llvm/bin"
 MAGIC = b"__CLANG_OFFLOAD_BUNDLE__"
 
 
+def _elf_section(path, name):
+    """The bytes of section `name` of the 64-bit little-endian ELF file at `path`."""
+    data = open(path, "rb").read()
+    assert data[:4] == b"\x7fELF" and data[4] == 2 and data[5] == 1, "not a 64-bit little-endian ELF"
+    shoff, = struct.unpack_from("<Q", data, 0x28)
+    shentsize, shnum, shstrndx = struct.unpack_from("<HHH", data, 0x3A)
+
+    def header(i):
+        # sh_name, sh_type, sh_flags, sh_addr, sh_offset, sh_size
+        return struct.unpack_from("<IIQQQQ", data, shoff + i * shentsize)
+    stroff = header(shstrndx)[4]
+    for i in range(shnum):
+        nm, _, _, _, off, size = header(i)
+        end = data.index(b"\0", stroff + nm)
+        if data[stroff + nm:end].decode() == name:
+            return data[off:off + size]
+    raise KeyError(name)
+
+
 def _code_objects(tmp_path):
     """The gfx950 code object of every offload bundle in the library's fat-binary
     section (one per translation unit with device code)."""
     lib = _engine.LIB_PATH
     if not os.path.exists(lib):
         pytest.fail(f"{lib} is missing: build it first")
-    for tool in ("objcopy",):
-        if not shutil.which(tool):
-            pytest.skip(f"{tool} not available")
     if not os.path.exists(os.path.join(LLVM, "clang-offload-bundler")):
         pytest.skip("clang-offload-bundler not available")
-    fatbin = tmp_path / "fatbin.bin"
-    subprocess.check_call(["objcopy", "--dump-section", f".hip_fatbin={fatbin}", lib])
-    data = fatbin.read_bytes()
+    data = _elf_section(lib, ".hip_fatbin")
     starts = []
     k = data.find(MAGIC)
     while k >= 0:
