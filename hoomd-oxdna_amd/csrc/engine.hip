@@ -818,19 +818,31 @@ void k_forces(ForceArgs a) {
         const double fene = (a.mask & (1u << T_BONDED)) ? s_fene[2 * slot] + s_fene[2 * slot + 1] : 0.0;
         a.split[i] = make_double4(0.5 * fene, 0.5 * acc[RS + 1], 0.5 * acc[RS + 2], 0.0);
     }
-    if (active && lane == 0) {
+    if constexpr (FUSED) {
+        // Langevin second half of step k with the force just computed, merged with the
+        // first half of step k + 1: both touch only particle i's own state, and the
+        // positions and frames of step k + 1 go to the other buffer of each pair, so no
+        // workgroup still reading step k sees them. The block's particles take this FP64
+        // sequence on the first wavefront, lane p for particle p (through LDS): on their
+        // group leaders it would be issued by every wavefront of the block.
+        if (active && lane == 0) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) s_rec[q][slot] = acc[q];
+        }
+        __syncthreads();
+        if (t < P && xcd_block() * P + t < a.N) {
+            const float4 fo = make_float4(s_rec[0][t], s_rec[1][t], s_rec[2][t], 0.f);
+            const float4 to = make_float4(s_rec[3][t], s_rec[4][t], s_rec[5][t], 0.f);
+            const double4 x = s_pos[t];
+            merged_particle_pred(a.st, xcd_block() * P + t, x, s_st[0][t], s_st[2][t], s_st[1][t], fo, to,
+                                 rebuild ? x : s_st[3][t], s_fr[0][t].w, CellSlot{s_bin[0][t], s_bin[1][t]});
+        }
+    } else if (active && lane == 0) {
         float ehalf = 0.f;
         if constexpr (NBASE > 6) ehalf = acc[6];
         float4 fo = make_float4(acc[0], acc[1], acc[2], ehalf);
         float4 to = make_float4(acc[3], acc[4], acc[5], 0.f);
-        if (FUSED) {
-            // Langevin second half of step k with the force just computed, merged with the
-            // first half of step k + 1: both touch only particle i's own state, and the
-            // positions and frames of step k + 1 go to the other buffer of each pair, so no
-            // workgroup still reading step k sees them
-            merged_particle_pred(a.st, i, pi, s_st[0][slot], s_st[2][slot], s_st[1][slot], fo, to,
-                                 rebuild ? pi : s_st[3][slot], charge_i, CellSlot{s_bin[0][slot], s_bin[1][slot]});
-        } else {
+        {
             if (a.accumulate) {
                 float4 f = a.force[i], tq = a.torque[i];
                 fo.x += f.x; fo.y += f.y; fo.z += f.z; fo.w += f.w;
