@@ -197,6 +197,25 @@ def test_bench_8k_workload_after_1000_graph_steps():
     _check_run_kernels(hot, "8k nt after 1000 steps", vel=vel, angmom=ang)
 
 
+@pytest.mark.parametrize("lanes", [4, 8, 32])
+def test_fused_run_step_other_layouts(lanes):
+    """The FUSED run-step launch at the other particles-per-workgroup layouts (64, 32 and
+    8 particles per workgroup; its epilogue integrates a workgroup's particles on the
+    first wavefront, one lane each) against the oracle: a 2,400-nt gas of 10-bp duplexes
+    (rows of at most ~21 entries, which 64 particles per workgroup hold in LDS; longer
+    rows make the engine take 32 lanes instead) after 300 thermal steps, with those
+    steps' momenta."""
+    system = systems.duplex_gas(120, 10, box=80.0, seed=11)
+    forces, _ = dna2.forces(fused=True)
+    s = sim.Simulation(device=sim.GPU(), seed=2, lanes_per_particle=lanes)
+    s.create_state_from_snapshot(system.snapshot())
+    s.operations.integrator = sim.Integrator(dt=DT, forces=forces, methods=[sim.Langevin(kT=KT)])
+    s.run(300)
+    pos, quat, vel, ang = _state(s)
+    _check_run_kernels(_at(system, pos, quat), f"2,400-nt gas after 300 steps, {lanes} lanes", vel=vel,
+                       angmom=ang, lanes=lanes)
+
+
 def test_bench_512_hairpins():
     reps = systems.hairpin_ensemble(512, seed=3, box=30.0)
     forces, _ = dna2.forces(fused=True)
