@@ -478,7 +478,9 @@ void k_forces(ForceArgs a) {
     __shared__ uint32_t s_cnt[P];  // rebuild launches: each particle's row length so far
     __shared__ uint32_t s_nn[P];   // row lengths
     __shared__ uint32_t s_off[3][P + 1];  // exclusive prefix sums of class-0 near / class-1 near / far counts
-    __shared__ float s_rec[NREC][256];
+    // FAR2: two rounds per chunk between barriers (their records side by side)
+    constexpr uint32_t RPC = FAR2 ? 2u : 1u, CH = 256u * RPC;
+    __shared__ float s_rec[NREC][CH];
     // The term parameters, staged in LDS: the inlined term evaluators reach them through
     // addresses the compiler may select per lane (one code path serving the hydrogen-
     // bonding and cross-stacking halves), which from global memory would be a vector
@@ -668,9 +670,21 @@ void k_forces(ForceArgs a) {
 #pragma unroll
     for (int k = 0; k < NREC; ++k) acc[k] = 0.f;
     const int nt = P_.ntypes;
+    // Row entry of thread t in round kk of the chunk at c0. FAR2 chunks hold two rounds
+    // with no barrier between them: a wavefront that finishes its first-round entries
+    // goes on to its second-round ones while the others still run. In the second round a
+    // wavefront takes the entry slot one below its own (wavefront 0, whose first-round
+    // entries, the class-0 ones, take longest, the last slot, usually the emptiest; the
+    // bond slots go to wavefront 1), so the rounds' chains even out across wavefronts.
+    auto entry_of = [&](uint32_t c0, uint32_t kk) -> uint32_t {
+        const uint32_t w = t / 64u;
+        const uint32_t sw = FAR2 ? ((w + 4u - kk) & 3u) : w;
+        return c0 + kk * 256u + sw * 64u + (t & 63u);
+    };
     // this thread's bond slot in the round that starts the bond slots: its partner and
     // role loaded before the rounds begin (the bond wavefront's first dependent load)
-    const uint32_t rb0 = B0 & ~255u, eb = rb0 + t;
+    const uint32_t rb0 = B0 & ~255u, kb = (rb0 / 256u) % RPC;
+    const uint32_t eb = entry_of(rb0 - kb * 256u, kb);
     const bool bpre = NB && eb >= B0 && eb < B0 + 2u * P;
     int b_other = -1, b_info = 0;
     if (bpre) {
@@ -680,14 +694,19 @@ void k_forces(ForceArgs a) {
             b_info = a.binfo[2 * ib + ((eb - B0) & 1u)];
         }
     }
-    // the wavefront after the class-0 entries does the integrator's force-free part
-    const uint32_t w_pre = min(3u, (NA + 63u) / 64u);
+    // the wavefront after the class-0 entries does the integrator's force-free part (FAR2:
+    // the one after that, as the next takes the bond slots in the second round)
+    const uint32_t w_pre = min(3u, (NA + 63u) / 64u + (FAR2 ? 1u : 0u));
     if (FUSED && NT == 0) {
         if (t < P) pre_particle(t);
         __syncthreads();
     }
-    for (uint32_t r0 = 0; r0 < NT; r0 += 256) {
-        const uint32_t e = r0 + t;
+    for (uint32_t c0 = 0; c0 < NT; c0 += CH) {
+#pragma unroll
+      for (uint32_t kk = 0; kk < RPC; ++kk) {
+        const uint32_t r0 = c0 + kk * 256u;  // this round's first entry
+        if (r0 >= NT) break;
+        const uint32_t e = entry_of(c0, kk);
         float rec[NREC];
 #pragma unroll
         for (int k = 0; k < NREC; ++k) rec[k] = 0.f;
@@ -785,13 +804,15 @@ void k_forces(ForceArgs a) {
             }
         }
 #pragma unroll
-        for (int q = 0; q < NREC; ++q) s_rec[q][t] = rec[q];
+        for (int q = 0; q < NREC; ++q) s_rec[q][e - c0] = rec[q];
         if (FUSED && r0 == 0 && t / 64u == w_pre && t % 64u < P) pre_particle(t % 64u);
+        if (r0 == 0 && late && RPC > 1) __builtin_amdgcn_s_setprio(0);
+      }
         __syncthreads();
-        if (r0 == 0 && late) __builtin_amdgcn_s_setprio(0);
-        // this particle's records of the round: near entries, bond slots, far entries
+        if (c0 == 0 && late && RPC == 1) __builtin_amdgcn_s_setprio(0);
+        // this particle's records of the chunk: near entries, bond slots, far entries
         if (active) {
-            const uint32_t r1 = r0 + 256;
+            const uint32_t r0 = c0, r1 = c0 + CH;
             uint32_t lo[4] = {s_off[0][slot], NA + s_off[1][slot], B0 + 2 * slot, F0 + s_off[2][slot]};
             uint32_t hi[4] = {s_off[0][slot + 1], NA + s_off[1][slot + 1], NB ? B0 + 2 * slot + 2 : B0 + 2 * slot,
                               F0 + s_off[2][slot + 1]};
