@@ -255,16 +255,17 @@ __device__ __forceinline__ void merged_particle(const StepArgs& a, uint32_t i, d
     step_finish(a, i, x, q, p, b, charge);
 }
 
-// merged_particle for the fused run step, whose particle already holds a cell slot bp
-// (taken during the rounds for the position predicted before the force kick): the slot
-// takes the particle if the new position is in that cell; otherwise it becomes a hole,
-// the particle is binned again, and the predicted cell goes to a.cells.rebin for its
-// clearing two steps later.
-__device__ __forceinline__ void merged_particle_pred(const StepArgs& a, uint32_t i, double4 x, double4 v,
-                                                     const double4& q0, double4 p, const float4& f, const float4& tq,
-                                                     const double4& plast, float charge, CellSlot bp) {
-    double4 q = q0;
-    lv_merge_kick(a.lv, f.x, f.y, f.z, tq.x, tq.y, tq.z, q, v, p);
+// merged_particle for the fused run step, in two independent parts for two wavefronts:
+// the translational one (the force's kick, drift, binning, skin test, cell member) and
+// the rotational one (the torque's kick, NO_SQUISH rotation, quaternion, angular momentum,
+// frames). The particle already holds a cell slot bp, taken during the rounds for the
+// position predicted before the force kick: the slot takes the particle if the new
+// position is in that cell; otherwise it becomes a hole, the particle is binned again,
+// and the predicted cell goes to a.cells.rebin for its clearing two steps later.
+__device__ __forceinline__ void merged_translate_pred(const StepArgs& a, uint32_t i, double4 x, double4 v,
+                                                      const float4& f, const double4& plast, CellSlot bp) {
+    const double h = 2.0 * a.lv.hm;
+    v.x += h * f.x; v.y += h * f.y; v.z += h * f.z;
     lv_drift(a.lv, a.L, a.Linv, v, x);
     a.pos_out[i] = x;
     a.vel[i] = v;
@@ -284,8 +285,28 @@ __device__ __forceinline__ void merged_particle_pred(const StepArgs& a, uint32_t
     const double dy = min_image(x.y - plast.y, a.L[1], a.Linv[1]);
     const double dz = min_image(x.z - plast.z, a.L[2], a.Linv[2]);
     if (dx * dx + dy * dy + dz * dz > a.skin_half2) atomicOr(a.rebuild_set, 1u);
+    if (!a.cells.count) return;
+    if (b.slot < a.cells.cap) {
+        a.cells.members[(size_t)b.cell * a.cells.cap + b.slot] = i;
+    } else {
+        atomicOr(&a.flags->overflow, 1u);
+        atomicMax(&a.flags->max_cell, b.slot + 1);
+    }
+}
+__device__ __forceinline__ void merged_rotate(const StepArgs& a, uint32_t i, double4 q, double4 p, const float4& tq,
+                                              float charge, float tbits) {
+    const double h2 = 2.0 * a.lv.dt;
+    const double4 T = qmul_vq(tq.x, tq.y, tq.z, q);
+    p.x += h2 * T.x; p.y += h2 * T.y; p.z += h2 * T.z; p.w += h2 * T.w;
     lv_rotate(a.lv, q, p);
-    step_finish(a, i, x, q, p, b, charge);
+    a.quat[i] = q;
+    a.angmom[i] = p;
+    float4 f0, f1, f2;
+    quat_axes(q.x, q.y, q.z, q.w, f0, f1, f2);
+    f0.w = charge;
+    f1.w = tbits;
+    f2.w = 0.f;
+    a.fr0[i] = f0; a.fr1[i] = f1; a.fr2[i] = f2;
 }
 
 // The per-particle state a run chunk starts from: what a chunk whose cells or rows
@@ -843,20 +864,28 @@ void k_forces(ForceArgs a) {
         // Langevin second half of step k with the force just computed, merged with the
         // first half of step k + 1: both touch only particle i's own state, and the
         // positions and frames of step k + 1 go to the other buffer of each pair, so no
-        // workgroup still reading step k sees them. The block's particles take this FP64
-        // sequence on the first wavefront, lane p for particle p (through LDS): on their
-        // group leaders it would be issued by every wavefront of the block.
+        // workgroup still reading step k sees them. The block's particles take these FP64
+        // sequences through LDS, lane p for particle p: the translational parts on the first
+        // wavefront and the rotational parts on the second, side by side (on their group
+        // leaders they would be issued, one after the other, by every wavefront).
         if (active && lane == 0) {
 #pragma unroll
             for (int q = 0; q < 6; ++q) s_rec[q][slot] = acc[q];
         }
         __syncthreads();
-        if (t < P && xcd_block() * P + t < a.N) {
-            const float4 fo = make_float4(s_rec[0][t], s_rec[1][t], s_rec[2][t], 0.f);
-            const float4 to = make_float4(s_rec[3][t], s_rec[4][t], s_rec[5][t], 0.f);
-            const double4 x = s_pos[t];
-            merged_particle_pred(a.st, xcd_block() * P + t, x, s_st[0][t], s_st[2][t], s_st[1][t], fo, to,
-                                 rebuild ? x : s_st[3][t], s_fr[0][t].w, CellSlot{s_bin[0][t], s_bin[1][t]});
+        const uint32_t pe = t & 63u;  // the particle of this lane of the first two wavefronts
+        if (t < 128u && pe < P && xcd_block() * P + pe < a.N) {
+            const uint32_t ip = xcd_block() * P + pe;
+            if (t < 64u) {
+                const float4 fo = make_float4(s_rec[0][pe], s_rec[1][pe], s_rec[2][pe], 0.f);
+                const double4 x = s_pos[pe];
+                merged_translate_pred(a.st, ip, x, s_st[0][pe], fo, rebuild ? x : s_st[3][pe],
+                                      CellSlot{s_bin[0][pe], s_bin[1][pe]});
+            } else {
+                const float4 to = make_float4(s_rec[3][pe], s_rec[4][pe], s_rec[5][pe], 0.f);
+                merged_rotate(a.st, ip, s_st[2][pe], s_st[1][pe], to, s_fr[0][pe].w,
+                              __int_as_float(type_of(s_pos[pe].w)));
+            }
         }
     } else if (active && lane == 0) {
         float ehalf = 0.f;
