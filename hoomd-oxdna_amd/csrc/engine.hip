@@ -553,20 +553,30 @@ void k_forces(ForceArgs a) {
     __shared__ double4 s_st[FUSED ? 4 : 1][FUSED ? P : 1];  // v, p, q, position of the last rebuild
     __shared__ uint32_t s_bin[2][FUSED ? P : 1];            // predicted cell and slot
     __shared__ double s_fene[WANT_E ? 2 * P : 1];           // FP64 FENE energy of each bond slot
-    auto pre_particle = [&](uint32_t ps) {
+    // in two independent parts, on two wavefronts: translation (velocity, cell clearing
+    // and the predicted cell slot) and rotation (angular momentum)
+    auto pre_rotate = [&](uint32_t ps) {
         const uint32_t ip = xcd_block() * P + ps;
         if (ip >= a.N) return;
         const StepArgs& s = a.st;
-        double4 v = s.vel[ip], p = s.angmom[ip];
+        double4 p = s.angmom[ip];
         const double4 q = s.quat[ip];
+        lv_merge_pre_rotate(s.lv, ip, *s.step_in, s.seed, q, p);
+        s_st[1][ps] = p; s_st[2][ps] = q;
+    };
+    auto pre_translate = [&](uint32_t ps) {
+        const uint32_t ip = xcd_block() * P + ps;
+        if (ip >= a.N) return;
+        const StepArgs& s = a.st;
+        double4 v = s.vel[ip];
         s_st[3][ps] = s.pos_last[ip];
         step_clear(s, s.pos_out[ip]);
         if (s.cells.count_prev) {
             const uint32_t rc = s.cells.rebin[ip];  // a mispredicted cell of two steps back
             if (rc < s.cells.ncell) s.cells.count_prev[rc] = 0u;
         }
-        lv_merge_pre(s.lv, ip, *s.step_in, s.seed, q, v, p);
-        s_st[0][ps] = v; s_st[1][ps] = p; s_st[2][ps] = q;
+        lv_merge_pre_translate(s.lv, ip, *s.step_in, s.seed, v);
+        s_st[0][ps] = v;
         // the cell slot for the position the drift gives before the force kick (the kick
         // moves it by dt^2 F / m): its atomic's round trip overlaps the rounds
         uint32_t pc = 0u, pslot = 0u;
@@ -715,11 +725,13 @@ void k_forces(ForceArgs a) {
             b_info = a.binfo[2 * ib + ((eb - B0) & 1u)];
         }
     }
-    // the wavefront after the class-0 entries does the integrator's force-free part (FAR2:
-    // the one after that, as the next takes the bond slots in the second round)
-    const uint32_t w_pre = min(3u, (NA + 63u) / 64u + (FAR2 ? 1u : 0u));
+    // the wavefronts after the class-0 entries do the integrator's force-free part,
+    // rotation and translation (FAR2: not the first of them, which takes the bond slots in
+    // the second round)
+    const uint32_t w_rot = min(3u, (NA + 63u) / 64u + (FAR2 ? 1u : 0u));
+    const uint32_t w_tra = w_rot < 3u ? w_rot + 1u : 2u;
     if (FUSED && NT == 0) {
-        if (t < P) pre_particle(t);
+        if (t < P) { pre_rotate(t); pre_translate(t); }
         __syncthreads();
     }
     for (uint32_t c0 = 0; c0 < NT; c0 += CH) {
@@ -826,7 +838,10 @@ void k_forces(ForceArgs a) {
         }
 #pragma unroll
         for (int q = 0; q < NREC; ++q) s_rec[q][e - c0] = rec[q];
-        if (FUSED && r0 == 0 && t / 64u == w_pre && t % 64u < P) pre_particle(t % 64u);
+        if (FUSED && r0 == 0 && t % 64u < P) {
+            if (t / 64u == w_rot) pre_rotate(t % 64u);
+            else if (t / 64u == w_tra) pre_translate(t % 64u);
+        }
         if (r0 == 0 && late && RPC > 1) __builtin_amdgcn_s_setprio(0);
       }
         __syncthreads();

@@ -232,6 +232,39 @@ __device__ __forceinline__ void lv_merge_pre(const Langevin& lv, uint32_t i, uns
     v.x += h * b[0]; v.y += h * b[1]; v.z += h * b[2];
     p.x += h2 * K.x; p.y += h2 * K.y; p.z += h2 * K.z; p.w += h2 * K.w;
 }
+// lv_merge_pre in two independent parts, for two lanes: the translational one (b from the
+// first Philox block) and the rotational one (rb from the first block's last word and the
+// second block). Each performs the same operations as lv_merge_pre on its quantities.
+__device__ __forceinline__ void lv_merge_pre_translate(const Langevin& lv, uint32_t i, unsigned long long step,
+                                                       unsigned long long seed, double4& v) {
+    uint32_t c[4] = {i, (uint32_t)step, (uint32_t)(step >> 32), 0u};
+    philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double ct = lv.ct;
+    const double b0 = -lv.gamma * v.x + ct * (2.0 * u01f(c[0]) - 1.0);
+    const double b1 = -lv.gamma * v.y + ct * (2.0 * u01f(c[1]) - 1.0);
+    const double b2 = -lv.gamma * v.z + ct * (2.0 * u01f(c[2]) - 1.0);
+    const double h = 2.0 * lv.hm;
+    v.x += h * b0; v.y += h * b1; v.z += h * b2;
+}
+__device__ __forceinline__ void lv_merge_pre_rotate(const Langevin& lv, uint32_t i, unsigned long long step,
+                                                    unsigned long long seed, const double4& q, double4& p) {
+    uint32_t c[4] = {i, (uint32_t)step, (uint32_t)(step >> 32), 0u};
+    philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint32_t d[4] = {i, (uint32_t)step, (uint32_t)(step >> 32), 1u};
+    philox(d, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double4 Lb = qmul(qconj(q), p);  // = (0, 2 L_body)
+    const double cr = lv.cr, gr = lv.gr_half_I;
+    float r1 = sqrtf(-2.f * __logf(u01f(c[3]))), r2 = sqrtf(-2.f * __logf(u01f(d[1])));
+    float s0, c0, s2, c2;
+    __sincosf(6.2831853f * u01f(d[0]), &s0, &c0);
+    __sincosf(6.2831853f * u01f(d[2]), &s2, &c2);
+    const double rb0 = cr * (double)(r1 * c0) - gr * Lb.y;
+    const double rb1 = cr * (double)(r1 * s0) - gr * Lb.z;
+    const double rb2 = cr * (double)(r2 * c2) - gr * Lb.w;
+    const double h2 = 2.0 * lv.dt;
+    const double4 K = qmul_qv(q, rb0, rb1, rb2);
+    p.x += h2 * K.x; p.y += h2 * K.y; p.z += h2 * K.z; p.w += h2 * K.w;
+}
 __device__ __forceinline__ void lv_merge_kick(const Langevin& lv, float fx, float fy, float fz, float tx, float ty,
                                               float tz, const double4& q, double4& v, double4& p) {
     const double h = 2.0 * lv.hm, h2 = 2.0 * lv.dt;
