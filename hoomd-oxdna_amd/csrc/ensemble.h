@@ -121,7 +121,7 @@ __host__ __device__ inline EnsLayout ens_layout(uint32_t ncap, uint32_t cap, uin
     l.acc = take(32 * ncap);                    // float4 [2][ncap]: force + energy, torque of earlier passes
     l.pass = take(8 * (ncap + 2));              // uint2 [passes + 1]: near / far scan bases
     l.red = take(8 * ENS_SLOTS * (nt / 64));    // per-wavefront energy sums
-    l.ctl = take(64);                           // pass count, bias record
+    l.ctl = take(64);                           // pass count, eval iteration counter, bias record
     // pair records, and one record of zeros at index cap
     l.rf = take(16 * (cap + 1));                // float4: F on the first end + pair energy
     l.rt = take(16 * (cap + 1));                // float4: torque on the first end (.w: list entry)
@@ -341,6 +341,7 @@ __global__ __launch_bounds__(NT) void k_ensemble(EnsArgs a) {
 #pragma unroll
         for (int s = 0; s < ENS_SLOTS; ++s) et[s] = 0.f;
         et_fene = 0.0;
+        if (t == 0) Sctl[1] = 0u;  // the eval's iteration counter (read after two barriers)
 
         // ---- test and bias. Test item (i, byte b) sets bits 8 (b % 4) .. of word b / 4 of
         //      particle i (rows d = 8 b + 1 .. 8 b + 8) and resets the incoming-record
@@ -525,24 +526,35 @@ __global__ __launch_bounds__(NT) void k_ensemble(EnsArgs a) {
                     fill(en, ef);
                     if (leader) Spt[p] = make_uint2(en | (ef << 16), pass);
                 }
+                if (t == 0) Sctl[1] = 0u;  // every wavefront left the last pass's eval
                 __syncthreads();
             }
 
             // ---- eval: (first pass) the bonds, then near entries, then far ones, in
-            //      whole-wavefront iterations dealt to the wavefronts from the last one (the
-            //      first ones lead the groups)
+            //      whole-wavefront iterations. A launch that reports energies deals them to
+            //      the wavefronts from the last one (the first ones lead the groups), so each
+            //      thread's energy sums have a fixed order; otherwise each wavefront takes
+            //      its next iteration from a counter in LDS, so those that draw cheap ones
+            //      (far entries) take more. A record's place does not depend on who writes
+            //      it, so the trajectory is the same either way.
             {
                 const uint32_t nn = pe.x - pb.x, nf = pe.y - pb.y;
                 const uint32_t In = (nn + 63u) / 64u, If = (nf + 63u) / 64u;
                 const uint32_t Ib = pass == 0 ? (nbond + 63u) / 64u : 0u;
+                const uint32_t Itot = Ib + In + If;
+                auto take = [&]() -> uint32_t {
+                    uint32_t x = 0u;
+                    if (lane == 0u) x = atomicAdd(&Sctl[1], 1u);
+                    return (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
+                };
                 // the bonds first: a bond's FP64 chain waits mostly on latency, which the
                 // other wavefront of its SIMD then fills with pairs
-                uint32_t it = NW - 1u - wv;
-                for (; it < Ib; it += NW) {
+                for (uint32_t it = want_e ? NW - 1u - wv : take(); it < Itot; it = want_e ? it + NW : take()) {
+                  if (it < Ib) {
                     // bonds of particle A where it is the A end; each end's share goes
                     // to the record of the slot that holds the bond
                     const uint32_t A = it * 64u + lane;
-                    if (A >= nbond) continue;
+                    if (A < nbond) {
                     const int4 ba = Sbond[A];
 #pragma unroll
                     for (int s = 0; s < 2; ++s) {
@@ -576,11 +588,11 @@ __global__ __launch_bounds__(NT) void k_ensemble(EnsArgs a) {
                         Sbrf[sB * ncap + B] = make_float4(br.FB.x, br.FB.y, br.FB.z, eh);
                         Sbrt[sB * ncap + B] = make_float4(br.TB.x, br.TB.y, br.TB.z, 0.f);
                     }
-                }
-                for (; it < Ib + In + If; it += NW) {
+                    }
+                  } else {
                     const bool near = it < Ib + In;
                     const uint32_t k = (near ? it - Ib : it - Ib - In) * 64u + lane;
-                    if (k >= (near ? nn : nf)) continue;
+                    if (k < (near ? nn : nf)) {
                     const uint32_t e = near ? k : cap - 1u - k;
                     const uint32_t packed = __float_as_uint(Rt[e].w);
                     const uint32_t i = packed & 0xffffu, d = packed >> 16;
@@ -604,6 +616,8 @@ __global__ __launch_bounds__(NT) void k_ensemble(EnsArgs a) {
                     Rf[e] = make_float4(Fp.x, Fp.y, Fp.z, ep);
                     Rt[e] = make_float4(Tp.x, Tp.y, Tp.z, 0.f);
                     Rj[e] = make_float4(Tj.x, Tj.y, Tj.z, 0.f);
+                    }
+                  }
                 }
             }
             __syncthreads();
