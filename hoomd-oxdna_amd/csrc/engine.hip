@@ -654,7 +654,6 @@ void k_forces(ForceArgs a) {
         nnear = min(ld_near & 0xffffu, nn);
         nA = min(ld_near >> 16, nnear);
     }
-    float charge_i = 1.f;
     if (lane == 0) {
         s_nnear[slot] = active ? nnear : 0u;
         s_off[0][slot + 1] = active ? nA : 0u;
@@ -665,7 +664,6 @@ void k_forces(ForceArgs a) {
             s_pos[slot] = pi;
             s_fr[0][slot] = ld_f0; s_fr[1][slot] = ld_f1; s_fr[2][slot] = ld_f2;
             s_head[slot] = rebuild ? (size_t)slot * 2 * nb.max_neigh + nb.max_neigh : ld_head;
-            charge_i = ld_f0.w;
         }
     }
     __syncthreads();
@@ -689,7 +687,8 @@ void k_forces(ForceArgs a) {
     const uint32_t NB = (mask & (1u << T_BONDED)) ? ((2u * P + 63u) & ~63u) : 0u;
     const uint32_t B0 = (NN + 63u) & ~63u, F0 = B0 + NB, NT = F0 + NF;
 
-    // ---- (2) entries, 256 per round; (3) each particle's lanes sum its records
+    // ---- (2) entries, 256 per round (FAR2: two rounds per barrier); (3) each particle's
+    //      lanes sum its records
     float et[8];
     double et_fene = 0.0;  // the FENE energy stays FP64 into the block reduction
     uint32_t cnt[14];
@@ -907,30 +906,28 @@ void k_forces(ForceArgs a) {
         if constexpr (NBASE > 6) ehalf = acc[6];
         float4 fo = make_float4(acc[0], acc[1], acc[2], ehalf);
         float4 to = make_float4(acc[3], acc[4], acc[5], 0.f);
-        {
-            if (a.accumulate) {
-                float4 f = a.force[i], tq = a.torque[i];
-                fo.x += f.x; fo.y += f.y; fo.z += f.z; fo.w += f.w;
-                to.x += tq.x; to.y += tq.y; to.z += tq.z;
-            }
-            a.force[i] = fo;
-            a.torque[i] = to;
-            if (WANT_V) {
+        if (a.accumulate) {
+            float4 f = a.force[i], tq = a.torque[i];
+            fo.x += f.x; fo.y += f.y; fo.z += f.z; fo.w += f.w;
+            to.x += tq.x; to.y += tq.y; to.z += tq.z;
+        }
+        a.force[i] = fo;
+        a.torque[i] = to;
+        if (WANT_V) {
 #pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    float v = acc[RV + q];
-                    if (a.accumulate) v += a.virial[q * a.vpitch + i];
-                    a.virial[q * a.vpitch + i] = v;
-                }
+            for (int q = 0; q < 6; ++q) {
+                float v = acc[RV + q];
+                if (a.accumulate) v += a.virial[q * a.vpitch + i];
+                a.virial[q * a.vpitch + i] = v;
             }
-            if (a.integrate) {
-                // the Langevin second half of step k alone (the last step of a run)
-                const StepArgs& s = a.st;
-                double4 v = s.vel[i], p = s.angmom[i], bd, bdt;
-                lv_second_half(s.lv, i, *s.step_in, s.seed, fo.x, fo.y, fo.z, to.x, to.y, to.z, s.quat[i], v, p,
-                               bd, bdt);
-                s.vel[i] = v; s.angmom[i] = p; s.bd[i] = bd; s.bd_tq[i] = bdt;
-            }
+        }
+        if (a.integrate) {
+            // the Langevin second half of step k alone (the last step of a run)
+            const StepArgs& s = a.st;
+            double4 v = s.vel[i], p = s.angmom[i], bd, bdt;
+            lv_second_half(s.lv, i, *s.step_in, s.seed, fo.x, fo.y, fo.z, to.x, to.y, to.z, s.quat[i], v, p,
+                           bd, bdt);
+            s.vel[i] = v; s.angmom[i] = p; s.bd[i] = bd; s.bd_tq[i] = bdt;
         }
     }
     if (FUSED && t == 0 && xcd_block() == 0) *a.st.step_out = *a.st.step_in + 1ull;
