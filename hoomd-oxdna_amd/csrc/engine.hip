@@ -668,7 +668,21 @@ void k_forces(ForceArgs a) {
     }
     __syncthreads();
     if constexpr (FAR2) asm volatile("" ::"s"(arg_lines));
-    if (t < 3) {  // exclusive prefix sums over the P particles, one thread per count
+    if constexpr (P <= 16) {
+        // exclusive prefix sums over the P particles: lane c P + q of the first wavefront
+        // holds particle q's count c, and the three groups of P lanes scan by shuffles
+        if (t < 3u * P) {
+            const uint32_t c = t / P, q = t % P;
+            uint32_t v = s_off[c][q + 1];
+#pragma unroll
+            for (uint32_t d = 1; d < (uint32_t)P; d <<= 1) {
+                const uint32_t u = (uint32_t)__shfl_up((int)v, d, 64);
+                if (q >= d) v += u;
+            }
+            s_off[c][q + 1] = v;
+            if (q == 0) s_off[c][0] = 0u;
+        }
+    } else if (t < 3) {  // exclusive prefix sums over the P particles, one thread per count
         uint32_t v[P];
 #pragma unroll
         for (int q = 0; q < P; ++q) v[q] = s_off[t][q + 1];
