@@ -319,22 +319,49 @@ struct RunState {
 // First half of step 1 of a run chunk from the stored state (step 0, the chunk's steps
 // numbered from step0), with the Langevin force and torque of the stored second half;
 // each particle's starting state is copied to `save` as it is read.
-__global__ void k_step1(StepArgs a, RunState save, unsigned long long step0) {
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i == 0) *a.step_out = step0 + 1ull;
+// A workgroup of kStep1Block threads covers kStep1Block / 2 particles: the first half of
+// its threads runs their translational part (kick, drift, binning, skin test, cell member),
+// the second half their rotational part (kick, NO_SQUISH rotation, frames), side by side
+// (the same operations as one thread doing both, so the same results).
+constexpr uint32_t kStep1Block = 256;
+__global__ __launch_bounds__(kStep1Block) void k_step1(StepArgs a, RunState save, unsigned long long step0) {
+    constexpr uint32_t H = kStep1Block / 2;
+    const bool rot = threadIdx.x >= H;
+    const uint32_t i = xcd_block() * H + (threadIdx.x % H);
+    if (i == 0 && !rot) *a.step_out = step0 + 1ull;
     if (i >= a.N) return;
-    double4 x = a.pos_in[i], v = a.vel[i], q = a.quat[i], p = a.angmom[i];
-    const float4 f = a.force[i], t = a.torque[i];
-    const double4 bd = a.bd[i], bdt = a.bd_tq[i], plast = a.pos_last[i];
-    save.pos[i] = x; save.vel[i] = v; save.quat[i] = q; save.angmom[i] = p;
-    save.force[i] = f; save.torque[i] = t; save.bd[i] = bd; save.bd_tq[i] = bdt; save.pos_last[i] = plast;
-    step_clear(a, a.pos_out[i]);
-    lv_first_half_translate(a.lv, a.L, a.Linv, f.x, f.y, f.z, bd, x, v);
-    a.pos_out[i] = x;
-    a.vel[i] = v;
-    const CellSlot b = step_bin(a, x, plast);
-    lv_first_half_rotate(a.lv, t.x, t.y, t.z, bdt, q, p);
-    step_finish(a, i, x, q, p, b, a.charge ? (float)a.charge[i] : 1.f);
+    if (!rot) {
+        double4 x = a.pos_in[i], v = a.vel[i];
+        const float4 f = a.force[i];
+        const double4 bd = a.bd[i], plast = a.pos_last[i];
+        save.pos[i] = x; save.vel[i] = v; save.force[i] = f; save.bd[i] = bd; save.pos_last[i] = plast;
+        step_clear(a, a.pos_out[i]);
+        lv_first_half_translate(a.lv, a.L, a.Linv, f.x, f.y, f.z, bd, x, v);
+        a.pos_out[i] = x;
+        a.vel[i] = v;
+        const CellSlot b = step_bin(a, x, plast);
+        if (!a.cells.count) return;
+        if (b.slot < a.cells.cap) {
+            a.cells.members[(size_t)b.cell * a.cells.cap + b.slot] = i;
+        } else {
+            atomicOr(&a.flags->overflow, 1u);
+            atomicMax(&a.flags->max_cell, b.slot + 1);
+        }
+    } else {
+        double4 q = a.quat[i], p = a.angmom[i];
+        const float4 t = a.torque[i];
+        const double4 bdt = a.bd_tq[i];
+        save.quat[i] = q; save.angmom[i] = p; save.torque[i] = t; save.bd_tq[i] = bdt;
+        lv_first_half_rotate(a.lv, t.x, t.y, t.z, bdt, q, p);
+        a.quat[i] = q;
+        a.angmom[i] = p;
+        float4 f0, f1, f2;
+        quat_axes(q.x, q.y, q.z, q.w, f0, f1, f2);
+        f0.w = a.charge ? (float)a.charge[i] : 1.f;
+        f1.w = __int_as_float(type_of(a.pos_in[i].w));
+        f2.w = 0.f;
+        a.fr0[i] = f0; a.fr1[i] = f1; a.fr2[i] = f2;
+    }
 }
 
 // Second half of step t - 1 and first half of step t, merged: the kernel between two
@@ -1910,7 +1937,8 @@ void run(oxdna_ctx* c, uint32_t nsteps, uint32_t mask, int use_graph) {
         const unsigned long long step0 = c->host_step + done;
         for (int attempt = 0;; ++attempt) {
             const size_t ncount = pairs ? 3 * ncell_total(c) : 0;
-            hipLaunchKernelGGL(k_step1, dim3(blocks_for(c->N)), dim3(kBlock), 0, c->stream, step_args(c, pairs, 1),
+            hipLaunchKernelGGL(k_step1, dim3((uint32_t)((c->N + kStep1Block / 2 - 1) / (kStep1Block / 2))),
+                               dim3(kStep1Block), 0, c->stream, step_args(c, pairs, 1),
                                saved_state(c), step0);
             uint32_t k = 0;  // inner steps enqueued
             const bool graphs = use_graph && chunk - 1 >= kGraphSteps;
